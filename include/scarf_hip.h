@@ -1,0 +1,172 @@
+/* scarf_hip.h — C ABI of libscarfhip.so, the MI355X (gfx950) kernels behind scarf's
+ * graph-building path (normalise -> mu/sigma -> PCA -> projection -> kNN ->
+ * smooth_knn_dist -> mapping scores).
+ *
+ * Conventions
+ * -----------
+ * - All array arguments are DEVICE pointers owned by the caller; sizes are element
+ *   counts.  Nothing here allocates device memory: entry points that need scratch
+ *   take a caller-provided workspace whose size the matching *_workspace() call gives.
+ * - Every call is ordered on the HIP stream passed as `stream` (a hipStream_t; NULL
+ *   means the default stream) and returns SH_OK (0) or a negative code; the message
+ *   of the last failure on the calling thread is available from sh_last_error().
+ * - Count matrices are CSR: u32 values, i32 column ids (sorted within a row), i64 row
+ *   pointers holding ABSOLUTE offsets into the value/column arrays, so a row chunk is
+ *   addressed by offsetting `rowptr` alone.
+ * - `col_map` (length n_cols_in, may be NULL for the identity) maps an input column to
+ *   its position among the d selected features, or -1 to ignore the column.  This is
+ *   the feature subset of scarf's `feat_key` and the feature alignment of mapping.
+ *
+ * The reference (scarf 0.32.3) has no native interface: its path is Python calling
+ * numpy/dask, scikit-learn IncrementalPCA, hnswlib and umap-learn.  Each entry point
+ * below names the reference code it replaces (paths relative to the scarf tree).
+ */
+#ifndef SCARF_HIP_H
+#define SCARF_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SH_OK 0
+#define SH_EINVAL (-1)      /* bad argument: shape, pointer, size, malformed CSR */
+#define SH_EHIP (-2)        /* HIP runtime error */
+#define SH_EWORKSPACE (-3)  /* workspace smaller than *_workspace() asked for */
+#define SH_EUNSUPPORTED (-4)
+
+/* Screening arithmetic for sh_knn_pack / sh_knn_screen. */
+#define SH_F32 0
+#define SH_BF16 1
+#define SH_FP8 2
+
+/* Bits of the per-query flags written by sh_knn_rerank / sh_knn_exact. */
+#define SH_FLAG_UNCERTIFIED 1 /* FP32 screen could not prove its top-k exact */
+#define SH_FLAG_SELF_MISSING 2 /* fix_knn_query had to drop the last neighbour */
+
+typedef void *sh_stream_t;
+
+int sh_version(void);
+const char *sh_last_error(void);
+
+/* ---- normalisation and feature statistics ------------------------------------
+ * Per-cell library size over the mapped columns (0 -> 1) unless `libsize_in` is
+ * given, x = log1p(sf * c / libsize) (or sf * c / libsize when log_transform == 0),
+ * and the chunk's per-feature FP64 sums of x and x^2 (implicit zeros contribute 0).
+ * Deterministic: fixed-order accumulation.
+ * Replaces: assay.py:812-828 (RNAassay.normed with renormalize_subset),
+ *           assay.py:41-64 (norm_lib_size[_log]),
+ *           graph_datastore.py:767-796 (data.mean(0) / data.std(0), per chunk). */
+int sh_norm_stats(const uint32_t *vals, const int32_t *cols, const int64_t *rowptr,
+                  int64_t n_rows, const int32_t *col_map, int32_t n_cols_in, int32_t d,
+                  double sf, int log_transform, const double *libsize_in,
+                  double *libsize_out, double *sum_out, double *sumsq_out,
+                  void *workspace, size_t workspace_bytes, sh_stream_t stream);
+size_t sh_norm_stats_workspace(int64_t n_rows, int32_t d);
+
+/* Merges per-chunk (count, sum, sumsq) [n_chunks][d] with Chan's pairwise formula
+ * into mu and sigma (population, ddof=0), then applies clean_array: NaN/inf -> 0,
+ * mu 0 stays 0, sigma 0 -> 1.  Optional per-chunk means [n_chunks][d].
+ * Replaces: graph_datastore.py:767-796 + utils.py:143-154 (clean_array). */
+int sh_stats_merge(const double *sums, const double *sumsqs, const int64_t *counts,
+                   int32_t n_chunks, int32_t d, double *mu_out, double *sigma_out,
+                   double *chunk_mean_out, sh_stream_t stream);
+
+/* Dense tile of z = (x - center) * scale (clipped to +-clip when clip > 0), FP64
+ * (out_f64 != 0) or FP32, for the rows `rows[0..n_rows)` (NULL: rows 0..n_rows-1 of
+ * the CSR); columns d..ld-1 are written as 0.  Feeds sh_gram.
+ * Replaces: ann.py:191-192 (transform_z) on the blocks fed to
+ *           IncrementalPCA.partial_fit (ann.py:229-241). */
+int sh_densify(const uint32_t *vals, const int32_t *cols, const int64_t *rowptr,
+               const int64_t *rows, int64_t n_rows, const int32_t *col_map,
+               int32_t n_cols_in, int32_t d, double sf, int log_transform,
+               const double *libsize, const double *center, const double *scale,
+               double clip, int out_f64, void *out, int64_t ld, sh_stream_t stream);
+
+/* G = Z^T Z (FP64 [d x d], full symmetric) of a dense Z [n x ld] on MFMA: FP64 Z on
+ * v_mfma_f64_16x16x4f64 (z_f64 != 0), or FP32 Z on v_mfma_f32_32x32x2f32 with FP32
+ * accumulation per row slab; slabs are summed in FP64 in a fixed order.  Optionally
+ * the FP64 column sums of Z (colsum_out [d], may be NULL).  accumulate != 0 adds to
+ * G_out / colsum_out instead of overwriting them.  ld must be a multiple of 128 with
+ * columns d..ld-1 zero (sh_densify writes them so).
+ * Replaces: the SVD inside IncrementalPCA.partial_fit (_incremental_pca.py:345-363,
+ *           called from ann.py:241,250): right singular vectors of the augmented
+ *           chunk are eigenvectors of its Gram matrix. */
+int sh_gram(const void *Z, int z_f64, int64_t n, int32_t d, int64_t ld, int accumulate,
+            double *G_out, double *colsum_out, void *workspace, size_t workspace_bytes,
+            sh_stream_t stream);
+size_t sh_gram_workspace(int64_t n, int32_t d, int z_f64);
+
+/* Sparse z-score projection: E[r] = sum_f z_rf W[f] with z = (x - mu) * inv_sigma
+ * (clipped to +-clip when clip > 0), evaluated as c + sum over the row's non-zeros of
+ * (z(x) - z(0)) * W[f], where c = sum_f z(0)_f W[f] (+ alignment fill, see mapping)
+ * is precomputed by the caller.  W is FP32 [d x ldw], E is FP32 [n_rows x lde].
+ * libsize NULL: computed over the mapped columns (mapping's target normalisation).
+ * Replaces: ann.py:129-138 (reducer = transform_z(x).dot(loadings)),
+ *           mapping_datastore.py:198-204 (target projection). */
+int sh_project(const uint32_t *vals, const int32_t *cols, const int64_t *rowptr,
+               int64_t n_rows, const int32_t *col_map, int32_t n_cols_in, int32_t d,
+               double sf, int log_transform, const double *libsize, const float *mu,
+               const float *inv_sigma, float clip, const float *W, int64_t ldw,
+               int32_t D, const float *c, float *E_out, int64_t lde, sh_stream_t stream);
+
+/* ---- k nearest neighbours (squared L2) ------------------------------------------
+ * Replaces hnswlib.Index (ann.py:14-28, 194-205, 290-326) with an exact screen:
+ *   1. sh_knn_pack       embedding rows -> MFMA operand layout (F32 / BF16 / FP8)
+ *   2. sh_knn_screen     ||r||^2 - 2 q.r tiles on MFMA; per (query, reference slice)
+ *                        the `pool` smallest (screened distance, index) pairs
+ *   3. sh_knn_rerank     merge of the slices, exact FP64 distances of the pool,
+ *                        top-k by (distance, lower index), fix_knn_query self repair
+ *                        (ann.py:31-52), and an error-bound certificate
+ *   4. sh_knn_exact      FP64 brute force for rows the certificate rejects (and the
+ *                        FP64 reference for recall measurements). */
+int32_t sh_knn_pack_k(int dtype, int32_t D);
+int sh_knn_pack(const float *E, int64_t n, int32_t D, int64_t lde, int dtype, int role,
+                void *out, float *aux_out, sh_stream_t stream);
+int sh_knn_screen(const void *qpack, int64_t nq, const void *rpack, int64_t nr,
+                  int32_t D, int dtype, const float *q_aux, const float *r_aux,
+                  int32_t pool, int32_t n_slices, uint64_t *cand_keys,
+                  int32_t *cand_cnt, void *workspace, size_t workspace_bytes,
+                  sh_stream_t stream);
+size_t sh_knn_screen_workspace(int64_t nq, int32_t pool, int32_t n_slices);
+int sh_knn_rerank(const float *Q, int64_t nq, int64_t ldq, const float *R, int64_t nr,
+                  int64_t ldr, int32_t D, const uint64_t *cand_keys,
+                  const int32_t *cand_cnt, int32_t pool, int32_t n_slices,
+                  int32_t k, int64_t self_offset, double err_coef, float r_norm_max,
+                  int64_t *idx_out, float *dist_out, int32_t *flags_out,
+                  sh_stream_t stream);
+int sh_knn_exact(const float *Q, int64_t nq, int64_t ldq, const int64_t *q_rows,
+                 int64_t n_sel, const float *R, int64_t nr, int64_t ldr, int32_t D,
+                 int32_t k, int64_t self_offset, int64_t *idx_out, float *dist_out,
+                 double *dist64_out, int32_t *flags_out, sh_stream_t stream);
+
+/* ---- fuzzy edge weights ---------------------------------------------------------
+ * umap smooth_knn_dist + compute_membership_strengths per `chunk_size` row chunk,
+ * then zero weights -> the global minimum rule of scarf.  dist FP32 [n x k],
+ * idx i64 [n x k]; outputs FP32 sigma/rho [n], weights [n * k] row-major.
+ * Replaces: knn_utils.py:89-159 (smoothen_dists). */
+int sh_smooth_knn(const float *dist, const int64_t *idx, int64_t n, int32_t k,
+                  int64_t chunk_size, float local_connectivity, float bandwidth,
+                  int32_t n_iter, float *sigma_out, float *rho_out, float *w_out,
+                  void *workspace, size_t workspace_bytes, sh_stream_t stream);
+size_t sh_smooth_knn_workspace(int64_t n, int64_t chunk_size);
+
+/* ---- mapping scores ---------------------------------------------------------------
+ * For each target group g: ms[ref] = sum over the group's (query, neighbour) pairs of
+ * 1 / (log1p(d) + 1) (or fixed_weight), scaled by multiplier / (n_g * k), then log1p.
+ * Accumulated in 2^-40 fixed point with integer atomics: order-independent and
+ * deterministic.  groups NULL: one group.  out FP64 [n_groups x n_ref].
+ * Replaces: mapping_datastore.py:211-285 (get_mapping_score). */
+int sh_mapping_score(const int64_t *idx, const float *dist, int64_t nq, int32_t k,
+                     const int32_t *groups, int32_t n_groups, int64_t n_ref,
+                     int weighted, double fixed_weight, double multiplier,
+                     int log_transform, double *out, void *workspace,
+                     size_t workspace_bytes, sh_stream_t stream);
+size_t sh_mapping_score_workspace(int64_t n_ref, int32_t n_groups);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SCARF_HIP_H */

@@ -1,0 +1,88 @@
+// Shared helpers for the scarf_amd HIP library (gfx950 / CDNA4, wave64).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+
+#include "scarf_hip.h"
+
+namespace sh {
+
+// Records a printf-style message for sh_last_error() (thread-local).
+void set_error(const char *fmt, ...);
+
+constexpr int kWave = 64;
+
+// Orderable 32-bit image of a float: a < b  <=>  ord(a) < ord(b) (for non-NaN).
+__device__ __forceinline__ uint32_t float_ord(float f) {
+  uint32_t u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float ord_float(uint32_t o) {
+  uint32_t u = (o & 0x80000000u) ? (o & 0x7fffffffu) : ~o;
+  return __uint_as_float(u);
+}
+// 64-bit sort key: distance first, reference index second (lower index wins ties).
+__device__ __forceinline__ uint64_t make_key(float d, uint32_t idx) {
+  return ((uint64_t)float_ord(d) << 32) | (uint64_t)idx;
+}
+__device__ __forceinline__ uint64_t double_ord(double d) {
+  uint64_t u = (uint64_t)__double_as_longlong(d);
+  return (u & 0x8000000000000000ull) ? ~u : (u | 0x8000000000000000ull);
+}
+
+template <typename T>
+__device__ __forceinline__ T shfl_xor(T v, int mask) {
+  return __shfl_xor(v, mask, kWave);
+}
+
+__device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int mask) {
+  uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
+  lo = __shfl_xor(lo, mask, kWave);
+  hi = __shfl_xor(hi, mask, kWave);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ double wave_sum_f64(double v) {
+  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, kWave);
+  return v;
+}
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
+  for (int m = 32; m >= 1; m >>= 1) v += shfl_xor_u64(v, m);
+  return v;
+}
+__device__ __forceinline__ float wave_max_f32(float v) {
+  for (int m = 32; m >= 1; m >>= 1) v = fmaxf(v, __shfl_xor(v, m, kWave));
+  return v;
+}
+
+inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+}  // namespace sh
+
+#define SH_ARG(cond, ...)                 \
+  do {                                    \
+    if (!(cond)) {                        \
+      ::sh::set_error(__VA_ARGS__);       \
+      return SH_EINVAL;                   \
+    }                                     \
+  } while (0)
+
+#define SH_HIP(call)                                                         \
+  do {                                                                       \
+    hipError_t e_ = (call);                                                  \
+    if (e_ != hipSuccess) {                                                  \
+      ::sh::set_error("%s failed: %s", #call, hipGetErrorString(e_));        \
+      return SH_EHIP;                                                        \
+    }                                                                        \
+  } while (0)
+
+#define SH_LAUNCH_CHECK(name)                                                \
+  do {                                                                       \
+    hipError_t e_ = hipGetLastError();                                       \
+    if (e_ != hipSuccess) {                                                  \
+      ::sh::set_error("%s launch failed: %s", name, hipGetErrorString(e_));   \
+      return SH_EHIP;                                                        \
+    }                                                                        \
+  } while (0)

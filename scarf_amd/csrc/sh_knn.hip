@@ -1,0 +1,726 @@
+// Exact k-nearest-neighbour search (squared L2) on MFMA with an error-bounded
+// low-precision screen.  Replaces scarf's hnswlib index (ann.py:14-28, 194-205,
+// 290-326) and fix_knn_query (ann.py:31-52).
+//
+// Screen: the MFMA computes s(q, r) = ||r||^2 - 2 q.r directly: reference rows are
+// packed as [r, ||r||^2 (split over 1 or 3 slots), 0...] and query rows as
+// [-2q, 1, ..., 0...], so one K-padded dot product gives the ranking key
+// (||q||^2 is the same for every reference of a query and is left out).
+//   A operand = 32 references, B operand = 32 queries, D = 32 x 32 FP32 tile: lane l
+//   holds query l%32 and 16 references 8*(v/4) + 4*(l/32) + v%4.
+// The K axis is split between the two lane halves: half h supplies elements
+// [h*K/2, (h+1)*K/2) of its row, for both operands, so each lane streams one
+// contiguous half-row (the MFMA pairs A and B by the same (half, element) slot,
+// so any such split gives the same dot product).
+//
+// Selection: per (query, reference slice) a candidate buffer of CAP (key, index)
+// pairs in global memory and a threshold tau = the pool-th best key so far.  A
+// 32 x 32 tile is filtered with a 16-value min tree and one compare per query;
+// only tiles with a hit (rare once tau has settled) take the insertion path.  When
+// a buffer nears CAP the owning wave sorts it (bitonic across the wave) and keeps
+// the best `pool`.  Each wave owns its queries, so no cross-wave synchronisation.
+#include <float.h>
+
+#include "sh_common.h"
+#include "sh_wave_sort.h"
+
+namespace sh {
+
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ uint16_t f2bf(float f) {  // round to nearest even
+  uint32_t u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+
+int32_t pack_k(int dtype, int32_t D) {
+  if (dtype == SH_F32) return (int32_t)align_up((size_t)D + 1, 8);
+  if (dtype == SH_BF16) return (int32_t)align_up((size_t)D + 3, 16);
+  return -1;
+}
+
+// ---------------------------------------------------------------------------------
+// Packing: one thread per row (rows are short: K <= 136 elements).
+__global__ void k_pack_f32(const float *__restrict__ E, int64_t n, int32_t D, int64_t lde,
+                           int role, int32_t K, float *__restrict__ out, float *__restrict__ aux) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const float *e = E + r * lde;
+  float *o = out + r * K;
+  double s = 0.0;
+  for (int t = 0; t < D; ++t) {
+    float x = e[t];
+    s += (double)x * (double)x;
+    o[t] = role == 0 ? x : -2.0f * x;
+  }
+  o[D] = role == 0 ? (float)s : 1.0f;
+  for (int t = D + 1; t < K; ++t) o[t] = 0.f;
+  if (aux) aux[r] = (float)s;
+}
+
+__global__ void k_pack_bf16(const float *__restrict__ E, int64_t n, int32_t D, int64_t lde,
+                            int role, int32_t K, uint16_t *__restrict__ out,
+                            float *__restrict__ aux) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const float *e = E + r * lde;
+  uint16_t *o = out + r * K;
+  double s = 0.0;
+  for (int t = 0; t < D; ++t) {
+    float x = e[t];
+    s += (double)x * (double)x;
+    o[t] = role == 0 ? f2bf(x) : f2bf(-2.0f * x);
+  }
+  if (role == 0) {
+    // ||r||^2 as an unevaluated sum of three bf16 values (24 significant bits)
+    float hi = bf2f(f2bf((float)s));
+    double r1 = s - (double)hi;
+    float mid = bf2f(f2bf((float)r1));
+    double r2 = r1 - (double)mid;
+    float lo = bf2f(f2bf((float)r2));
+    o[D] = f2bf(hi);
+    o[D + 1] = f2bf(mid);
+    o[D + 2] = f2bf(lo);
+  } else {
+    o[D] = o[D + 1] = o[D + 2] = f2bf(1.0f);
+  }
+  for (int t = D + 3; t < K; ++t) o[t] = 0;
+  if (aux) aux[r] = (float)s;
+}
+
+// ---------------------------------------------------------------------------------
+// Screen kernel.
+struct OpF32 {
+  typedef float Frag;
+  static constexpr int EPL = 1;  // elements per lane per MFMA
+  static constexpr int ESZ = 4;
+  static __device__ __forceinline__ floatx16 mma(Frag a, Frag b, floatx16 c) {
+    return __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, c, 0, 0, 0);
+  }
+};
+struct OpBF16 {
+  typedef bf16x8 Frag;
+  static constexpr int EPL = 8;
+  static constexpr int ESZ = 2;
+  static __device__ __forceinline__ floatx16 mma(Frag a, Frag b, floatx16 c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+  }
+};
+
+constexpr int kQG = 2;                 // 32-query groups per wave
+constexpr int kQPW = 32 * kQG;         // queries per wave
+constexpr int kQPB = 4 * kQPW;         // queries per workgroup
+constexpr int kRT = 64;                // references per LDS stage
+
+template <int CAP>
+__device__ __forceinline__ void compact_query(uint64_t *qbuf, int *cnt_slot, float *tau_slot,
+                                              int pool, uint64_t *out, int32_t *out_cnt) {
+  constexpr int E = CAP / 64;
+  const int lane = threadIdx.x & 63;
+  const int c = *cnt_slot;  // uniform
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+  uint64_t v[E];
+#pragma unroll
+  for (int i = 0; i < E; ++i) {
+    const int e = lane * E + i;
+    v[i] = e < c ? qbuf[e] : ~0ull;
+  }
+  wave_bitonic_sort<E>(v);
+  const int keep = c < pool ? c : pool;
+  uint64_t *dst = out ? out : qbuf;
+#pragma unroll
+  for (int i = 0; i < E; ++i) {
+    const int e = lane * E + i;
+    if (e < keep) dst[e] = v[i];
+  }
+  const uint64_t kp = wave_elem_u64<E>(v, pool - 1);
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+  if (lane == 0) {
+    *cnt_slot = keep;
+    *tau_slot = c >= pool ? ord_float((uint32_t)(kp >> 32)) : INFINITY;
+    if (out_cnt) *out_cnt = keep;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+}
+
+template <class Op, int K, int CAP>
+__global__ void __launch_bounds__(256) k_screen(
+    const typename Op::Frag *__restrict__ Qp_, int64_t nq, const unsigned char *__restrict__ Rp,
+    int64_t nr, int64_t slice_len, int S, int pool, uint64_t *__restrict__ buf,
+    uint64_t *__restrict__ out, int32_t *__restrict__ out_cnt) {
+  typedef typename Op::Frag Frag;
+  constexpr int HALF = K / 2;                 // elements per lane half
+  constexpr int NS = HALF / Op::EPL;          // MFMA steps per tile
+  constexpr int ROWB = K * Op::ESZ + 16;      // padded LDS row bytes
+  constexpr int CHUNKS = K * Op::ESZ / 16;    // 16-byte chunks per packed row
+  __shared__ __attribute__((aligned(16))) unsigned char lds_r[kRT * ROWB];
+  __shared__ int cnt_lds[kQPB];
+  __shared__ float tau_lds[kQPB];
+
+  const unsigned char *Qp = reinterpret_cast<const unsigned char *>(Qp_);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, h = lane >> 5, n = lane & 31;
+  const int slice = blockIdx.y;
+  const int64_t r_begin = (int64_t)slice * slice_len;
+  const int64_t r_end = min(nr, r_begin + slice_len);
+  const int qw0 = w * kQPW;  // wave's first local query
+  const int64_t qb0 = (int64_t)blockIdx.x * kQPB;
+
+  Frag bq[kQG][NS];
+#pragma unroll
+  for (int g = 0; g < kQG; ++g) {
+    const int64_t q = qb0 + qw0 + g * 32 + n;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      if (q < nq)
+        bq[g][s] = *reinterpret_cast<const Frag *>(Qp + (q * K + h * HALF + s * Op::EPL) * Op::ESZ);
+      else
+        bq[g][s] = Frag{};
+    }
+  }
+  if (h == 0) {
+#pragma unroll
+    for (int g = 0; g < kQG; ++g) {
+      cnt_lds[qw0 + g * 32 + n] = 0;
+      tau_lds[qw0 + g * 32 + n] = INFINITY;
+    }
+  }
+  float tau[kQG];
+  bool qvalid[kQG];
+#pragma unroll
+  for (int g = 0; g < kQG; ++g) {
+    tau[g] = INFINITY;
+    qvalid[g] = qb0 + qw0 + g * 32 + n < nq;
+  }
+
+  for (int64_t t0 = r_begin; t0 < r_end; t0 += kRT) {
+    __syncthreads();
+    for (int e = threadIdx.x; e < kRT * CHUNKS; e += 256) {
+      const int rr = e / CHUNKS, c = e % CHUNKS;
+      const int64_t r = t0 + rr;
+      uint4 v = make_uint4(0, 0, 0, 0);
+      if (r < r_end) v = *reinterpret_cast<const uint4 *>(Rp + (r * K) * Op::ESZ + c * 16);
+      *reinterpret_cast<uint4 *>(lds_r + rr * ROWB + c * 16) = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int sub = 0; sub < kRT / 32; ++sub) {
+      const int64_t rb = t0 + sub * 32;
+      if (rb >= r_end) break;
+      floatx16 acc[kQG];
+#pragma unroll
+      for (int g = 0; g < kQG; ++g)
+#pragma unroll
+        for (int v = 0; v < 16; ++v) acc[g][v] = 0.f;
+      const unsigned char *arow = lds_r + (sub * 32 + n) * ROWB + h * HALF * Op::ESZ;
+#pragma unroll
+      for (int s = 0; s < NS; ++s) {
+        Frag a = *reinterpret_cast<const Frag *>(arow + s * Op::EPL * Op::ESZ);
+#pragma unroll
+        for (int g = 0; g < kQG; ++g) acc[g] = Op::mma(a, bq[g][s], acc[g]);
+      }
+      if (rb + 32 > r_end) {  // references past the slice end: NaN never passes
+#pragma unroll
+        for (int g = 0; g < kQG; ++g)
+#pragma unroll
+          for (int v = 0; v < 16; ++v) {
+            const int64_t ref = rb + 8 * (v >> 2) + 4 * h + (v & 3);
+            if (ref >= r_end) acc[g][v] = __builtin_nanf("");
+          }
+      }
+      bool hit = false;
+#pragma unroll
+      for (int g = 0; g < kQG; ++g) {
+        float m = acc[g][0];
+#pragma unroll
+        for (int v = 1; v < 16; ++v) m = fminf(m, acc[g][v]);
+        hit |= qvalid[g] && (m <= tau[g]);
+      }
+      if (__any(hit)) {
+#pragma unroll
+        for (int g = 0; g < kQG; ++g) {
+          const int ql = qw0 + g * 32 + n;
+          const int64_t q = qb0 + ql;
+          uint64_t *qbuf = buf + ((q < nq ? q : 0) * S + slice) * CAP;
+#pragma unroll
+          for (int v = 0; v < 16; ++v) {
+            if (q < nq && acc[g][v] <= tau[g]) {
+              const int64_t ref = rb + 8 * (v >> 2) + 4 * h + (v & 3);
+              const int pos = atomicAdd(&cnt_lds[ql], 1);
+              qbuf[pos] = make_key(acc[g][v], (uint32_t)ref);
+            }
+          }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        // compact every query of this wave whose buffer cannot take another tile
+#pragma unroll
+        for (int g = 0; g < kQG; ++g) {
+          const int ql = qw0 + g * 32 + n;
+          bool need = cnt_lds[ql] > CAP - 32;
+          uint64_t m = __ballot(need);
+          while (m) {
+            const int src = __ffsll((unsigned long long)m) - 1;
+            m &= m - 1;
+            if (src >= 32) continue;  // lanes n and n+32 share a query
+            const int qls = qw0 + g * 32 + src;
+            const int64_t qs = qb0 + qls;
+            compact_query<CAP>(buf + (qs * S + slice) * CAP, &cnt_lds[qls], &tau_lds[qls], pool,
+                               nullptr, nullptr);
+          }
+          tau[g] = tau_lds[ql];
+        }
+      }
+    }
+  }
+  // final: sort each query's buffer and emit its best `pool`
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+  for (int j = 0; j < kQPW; ++j) {
+    const int ql = qw0 + j;
+    const int64_t q = qb0 + ql;
+    if (q >= nq) break;
+    compact_query<CAP>(buf + (q * S + slice) * CAP, &cnt_lds[ql], &tau_lds[ql], pool,
+                       out + (q * S + slice) * pool, out_cnt + q * S + slice);
+  }
+}
+
+// ---------------------------------------------------------------------------------
+// Exact FP64 squared distance, terms added in feature order, no contraction: the
+// same evaluation as the CPU oracle, so results agree bit for bit.
+__device__ __forceinline__ double sqdist_exact(const float *__restrict__ q_lds,
+                                               const float *__restrict__ r, int D) {
+#pragma clang fp contract(off)
+  double acc = 0.0;
+  for (int t = 0; t < D; ++t) {
+    double df = (double)q_lds[t] - (double)r[t];
+    double sq = df * df;
+    acc = acc + sq;
+  }
+  return acc;
+}
+
+// One wave per query: merge the slices' pools to the global best `pool` by screened
+// key, re-rank them exactly, take the best k (+1 with self repair).
+template <int PE>  // pool elements per lane (pool <= 64 * PE)
+__global__ void __launch_bounds__(256) k_rerank(
+    const float *__restrict__ Q, int64_t nq, int64_t ldq, const float *__restrict__ R,
+    int64_t nr, int64_t ldr, int32_t D, const uint64_t *__restrict__ cand,
+    const int32_t *__restrict__ cand_cnt, int pool, int S, int k, int64_t self_offset,
+    double err_coef, float r_norm_max, int64_t *__restrict__ idx_out,
+    float *__restrict__ dist_out, int32_t *__restrict__ flags) {
+  __shared__ float qv[4][256];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int64_t q = (int64_t)blockIdx.x * 4 + w;
+  if (q >= nq) return;
+  float *ql = qv[w];
+  for (int t = lane; t < D; t += 64) ql[t] = Q[q * ldq + t];
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+
+  // 1. merged pool: best `pool` keys over the S slices
+  uint64_t v[PE];
+  int total = 0;
+  {
+    const int c0 = cand_cnt[q * S];
+    const uint64_t *p0 = cand + (q * S) * pool;
+#pragma unroll
+    for (int i = 0; i < PE; ++i) {
+      const int e = lane * PE + i;
+      v[i] = e < c0 ? p0[e] : ~0ull;
+    }
+    total = c0;
+  }
+  for (int s = 1; s < S; ++s) {
+    const int cs = cand_cnt[q * S + s];
+    const uint64_t *ps = cand + (q * S + s) * pool;
+    uint64_t m[2 * PE];
+#pragma unroll
+    for (int i = 0; i < PE; ++i) m[i] = v[i];
+    // second half of the 2*PE-per-lane array: the new slice
+#pragma unroll
+    for (int i = 0; i < PE; ++i) {
+      const int e = lane * PE + i;
+      m[PE + i] = e < cs ? ps[e] : ~0ull;
+    }
+    wave_bitonic_sort<2 * PE>(m);
+    // the best `pool` now sit at elements 0..pool-1 = lanes 0..pool/(2PE)-1; move
+    // them back to the PE-per-lane layout: element e lives at lane e/(2PE), slot e%(2PE)
+#pragma unroll
+    for (int i = 0; i < PE; ++i) {
+      const int e = lane * PE + i;
+      const int src = e / (2 * PE), slot = e % (2 * PE);
+      uint64_t mine = 0;
+      // every lane offers its slot `slot`... gather with a shuffle per slot value
+#pragma unroll
+      for (int sl = 0; sl < 2 * PE; ++sl) {
+        uint32_t lo = __shfl((uint32_t)m[sl], src, kWave);
+        uint32_t hi = __shfl((uint32_t)(m[sl] >> 32), src, kWave);
+        if (sl == slot) mine = ((uint64_t)hi << 32) | lo;
+      }
+      v[i] = mine;
+    }
+    total += cs;
+  }
+  const int npool = total < pool ? total : pool;
+  // screened key of the pool's last element: the bound on everything left out
+  const uint64_t lastkey = wave_elem_u64<PE>(v, npool > 0 ? npool - 1 : 0);
+
+  // 2. exact distances of the pool
+  KeyIdx c[PE];
+#pragma unroll
+  for (int i = 0; i < PE; ++i) {
+    const int e = lane * PE + i;
+    if (e < npool) {
+      const uint32_t j = (uint32_t)(v[i] & 0xffffffffu);
+      c[i].d = sqdist_exact(ql, R + (int64_t)j * ldr, D);
+      c[i].i = j;
+    } else {
+      c[i].d = INFINITY;
+      c[i].i = 0xffffffffu;
+    }
+  }
+  wave_bitonic_sort<PE>(c);
+
+  const bool self = self_offset >= 0;
+  const int kk = self ? k + 1 : k;
+  // broadcast the first kk (<= 64*PE) sorted elements through LDS-free shuffles
+  // element e is at lane e/PE, slot e%PE
+  int32_t flag = 0;
+  // certificate: every reference outside the pool has exact distance
+  // >= lastkey + ||q||^2 - err; the k-th chosen one must be strictly below that
+  if (err_coef >= 0.0) {
+    bool certified = true;
+    if (total > pool && kk > 0) {
+      double qn = 0.0;
+      for (int t = 0; t < D; ++t) qn += (double)ql[t] * (double)ql[t];
+      const double rq = sqrt(qn);
+      const double err =
+          err_coef * ((double)r_norm_max * r_norm_max + 2.0 * rq * (double)r_norm_max + qn);
+      const double bound = (double)ord_float((uint32_t)(lastkey >> 32)) + qn - err;
+      // kk-th exact distance (element kk-1)
+      const int e = (kk <= npool ? kk : npool) - 1;
+      double dk = 0.0;
+      {
+        double mine = 0.0;
+#pragma unroll
+        for (int i = 0; i < PE; ++i)
+          if (lane * PE + i == e) mine = c[i].d;
+        dk = __shfl(mine, e / PE, kWave);
+      }
+      certified = dk < bound;
+    }
+    if (!certified) flag |= SH_FLAG_UNCERTIFIED;
+  }
+  // gather the top kk (index, distance) into lane-indexed registers: lane t <- elem t
+  double dd[2];
+  uint32_t ii[2];
+#pragma unroll
+  for (int rep = 0; rep < 2; ++rep) {
+    const int e = lane + 64 * rep;
+    const int src = (e < 64 * PE ? e : 0) / PE, slot = e % PE;
+    double dm = INFINITY;
+    uint32_t im = 0xffffffffu;
+#pragma unroll
+    for (int sl = 0; sl < PE; ++sl) {
+      double t_d = __shfl(c[sl].d, src, kWave);
+      uint32_t t_i = (uint32_t)__shfl((int)c[sl].i, src, kWave);
+      if (sl == slot) {
+        dm = t_d;
+        im = t_i;
+      }
+    }
+    if (e >= npool) {
+      dm = INFINITY;
+      im = 0xffffffffu;
+    }
+    dd[rep] = dm;
+    ii[rep] = im;
+  }
+  // self repair (ann.py:31-52) on the kk = k+1 list
+  int drop = -1;
+  if (self) {
+    const int64_t me = self_offset + q;
+    int pos = -1;
+    for (int e = 0; e < kk; ++e) {
+      const uint32_t ie = (uint32_t)__shfl((int)ii[e / 64], e % 64, kWave);
+      if ((int64_t)ie == me) {
+        pos = e;
+        break;
+      }
+    }
+    if (pos < 0) {
+      drop = kk - 1;
+      flag |= SH_FLAG_SELF_MISSING;
+    } else {
+      drop = pos;
+    }
+  }
+  // output column col takes element col (col < drop) or col + 1 (col >= drop); the
+  // element may sit in another lane: shuffle
+#pragma unroll
+  for (int rep = 0; rep < 2; ++rep) {
+    const int col = lane + 64 * rep;
+    const int e = (drop >= 0 && col >= drop) ? col + 1 : col;
+    const int esrc = e < 128 ? e : 127;
+    double d0 = __shfl(dd[0], esrc % 64, kWave), d1 = __shfl(dd[1], esrc % 64, kWave);
+    uint32_t i0 = (uint32_t)__shfl((int)ii[0], esrc % 64, kWave);
+    uint32_t i1 = (uint32_t)__shfl((int)ii[1], esrc % 64, kWave);
+    const double de = esrc < 64 ? d0 : d1;
+    const uint32_t ie = esrc < 64 ? i0 : i1;
+    if (col < k) {
+      idx_out[q * k + col] = ie == 0xffffffffu ? -1 : (int64_t)ie;
+      dist_out[q * k + col] = (float)de;
+    }
+  }
+  if (flags && lane == 0) flags[q] = flag;
+}
+
+// FP64 brute force over all references for selected query rows (rows the
+// certificate rejected, or every row when used as the FP64 reference).  One wave per
+// query; each lane keeps a sorted top-KK of the references it visits
+// (lane, lane + 64, ...), then the wave sorts the 64 * KK survivors.
+template <int KK>
+__global__ void __launch_bounds__(256) k_exact(
+    const float *__restrict__ Q, int64_t ldq, const int64_t *__restrict__ q_rows, int64_t n_sel,
+    const float *__restrict__ R, int64_t nr, int64_t ldr, int32_t D, int k, int64_t self_offset,
+    int64_t *__restrict__ idx_out, float *__restrict__ dist_out, double *__restrict__ dist64_out,
+    int32_t *__restrict__ flags) {
+  __shared__ float qv[4][256];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int64_t sel = (int64_t)blockIdx.x * 4 + w;
+  if (sel >= n_sel) return;
+  const int64_t q = q_rows ? q_rows[sel] : sel;
+  float *ql = qv[w];
+  for (int t = lane; t < D; t += 64) ql[t] = Q[q * ldq + t];
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  KeyIdx best[KK];
+#pragma unroll
+  for (int i = 0; i < KK; ++i) {
+    best[i].d = INFINITY;
+    best[i].i = 0xffffffffu;
+  }
+  for (int64_t j = lane; j < nr; j += 64) {
+    KeyIdx x;
+    x.d = sqdist_exact(ql, R + j * ldr, D);
+    x.i = (uint32_t)j;
+    if (less_ki(x, best[KK - 1])) {
+#pragma unroll
+      for (int p = KK - 1; p >= 0; --p) {
+        const bool here = less_ki(x, best[p]);
+        const bool prev = p > 0 && less_ki(x, best[p - 1]);
+        if (here) best[p] = prev ? best[p - 1] : x;
+      }
+    }
+  }
+  wave_bitonic_sort<KK>(best);
+  const bool self = self_offset >= 0;
+  const int kk = self ? k + 1 : k;
+  int drop = -1;
+  int32_t flag = 0;
+  if (self) {
+    const int64_t me = self_offset + q;
+    int pos = -1;
+    for (int e = 0; e < kk; ++e) {
+      uint32_t mine = 0;
+#pragma unroll
+      for (int i = 0; i < KK; ++i)
+        if (lane * KK + i == e) mine = best[i].i;
+      const uint32_t ie = (uint32_t)__shfl((int)mine, e / KK, kWave);
+      if ((int64_t)ie == me) {
+        pos = e;
+        break;
+      }
+    }
+    drop = pos < 0 ? kk - 1 : pos;
+    if (pos < 0) flag |= SH_FLAG_SELF_MISSING;
+  }
+  for (int col = 0; col < k; ++col) {
+    const int e = (drop >= 0 && col >= drop) ? col + 1 : col;
+    double md = 0.0;
+    uint32_t mi = 0;
+#pragma unroll
+    for (int i = 0; i < KK; ++i)
+      if (lane * KK + i == e) {
+        md = best[i].d;
+        mi = best[i].i;
+      }
+    const double de = __shfl(md, e / KK, kWave);
+    const uint32_t ie = (uint32_t)__shfl((int)mi, e / KK, kWave);
+    if (lane == 0) {
+      idx_out[q * k + col] = ie == 0xffffffffu ? -1 : (int64_t)ie;
+      dist_out[q * k + col] = (float)de;
+      if (dist64_out) dist64_out[q * k + col] = de;
+    }
+  }
+  if (flags && lane == 0) flags[q] = flag;
+}
+
+}  // namespace sh
+
+using namespace sh;
+
+extern "C" int32_t sh_knn_pack_k(int dtype, int32_t D) { return pack_k(dtype, D); }
+
+extern "C" int sh_knn_pack(const float *E, int64_t n, int32_t D, int64_t lde, int dtype, int role,
+                           void *out, float *aux_out, sh_stream_t stream) {
+  SH_ARG(n >= 0 && D > 0 && D <= 128 && lde >= D, "sh_knn_pack: bad sizes D=%d lde=%lld", D,
+         (long long)lde);
+  SH_ARG(role == 0 || role == 1, "sh_knn_pack: role must be 0 (reference) or 1 (query)");
+  SH_ARG(E && out, "sh_knn_pack: NULL pointer");
+  const int32_t K = pack_k(dtype, D);
+  if (K < 0) {
+    set_error("sh_knn_pack: unsupported dtype %d", dtype);
+    return SH_EUNSUPPORTED;
+  }
+  if (n == 0) return SH_OK;
+  dim3 grid(cdiv(n, 256)), block(256);
+  if (dtype == SH_F32)
+    hipLaunchKernelGGL(k_pack_f32, grid, block, 0, (hipStream_t)stream, E, n, D, lde, role, K,
+                       (float *)out, aux_out);
+  else
+    hipLaunchKernelGGL(k_pack_bf16, grid, block, 0, (hipStream_t)stream, E, n, D, lde, role, K,
+                       (uint16_t *)out, aux_out);
+  SH_LAUNCH_CHECK("k_pack");
+  return SH_OK;
+}
+
+static int screen_cap(int pool) { return pool <= 64 ? 256 : 512; }
+
+extern "C" size_t sh_knn_screen_workspace(int64_t nq, int32_t pool, int32_t n_slices) {
+  return (size_t)nq * n_slices * screen_cap(pool) * sizeof(uint64_t);
+}
+
+template <class Op, int K, int CAP>
+static void launch_screen(dim3 grid, hipStream_t st, const void *qpack, int64_t nq,
+                          const void *rpack, int64_t nr, int64_t slice_len, int S, int pool,
+                          uint64_t *buf, uint64_t *out, int32_t *cnt) {
+  hipLaunchKernelGGL((k_screen<Op, K, CAP>), grid, dim3(256), 0, st,
+                     (const typename Op::Frag *)qpack, nq, (const unsigned char *)rpack, nr,
+                     slice_len, S, pool, buf, out, cnt);
+}
+
+extern "C" int sh_knn_screen(const void *qpack, int64_t nq, const void *rpack, int64_t nr,
+                             int32_t D, int dtype, const float *q_aux, const float *r_aux,
+                             int32_t pool, int32_t n_slices, uint64_t *cand_keys,
+                             int32_t *cand_cnt, void *workspace, size_t workspace_bytes,
+                             sh_stream_t stream) {
+  (void)q_aux;
+  (void)r_aux;
+  SH_ARG(nq >= 0 && nr > 0 && D > 0, "sh_knn_screen: bad sizes nq=%lld nr=%lld D=%d",
+         (long long)nq, (long long)nr, D);
+  SH_ARG(nr < 0xffffffffll, "sh_knn_screen: more than 2^32-1 references");
+  SH_ARG(pool >= 1 && pool <= 128, "sh_knn_screen: pool=%d outside [1, 128]", pool);
+  SH_ARG(n_slices >= 1 && n_slices <= 4096, "sh_knn_screen: n_slices=%d", n_slices);
+  SH_ARG(qpack && rpack && cand_keys && cand_cnt, "sh_knn_screen: NULL pointer");
+  const size_t need = sh_knn_screen_workspace(nq, pool, n_slices);
+  if (!workspace || workspace_bytes < need) {
+    set_error("sh_knn_screen: workspace %zu < %zu bytes", workspace_bytes, need);
+    return SH_EWORKSPACE;
+  }
+  if (nq == 0) return SH_OK;
+  const int32_t K = pack_k(dtype, D);
+  int64_t slice_len = align_up((size_t)cdiv(nr, n_slices), kRT);
+  const int S = (int)cdiv(nr, slice_len);
+  SH_ARG(S == n_slices, "sh_knn_screen: %d slices requested but %lld references give %d; use "
+         "a slice count whose slices hold >= 64 references", n_slices, (long long)nr, S);
+  dim3 grid(cdiv(nq, kQPB), S);
+  hipStream_t st = (hipStream_t)stream;
+  uint64_t *buf = (uint64_t *)workspace;
+  const bool big = pool > 64;
+#define SH_SCREEN_CASE(OP, KV)                                                               \
+  if (K == KV) {                                                                             \
+    if (big)                                                                                 \
+      launch_screen<OP, KV, 512>(grid, st, qpack, nq, rpack, nr, slice_len, S, pool, buf,    \
+                                 cand_keys, cand_cnt);                                       \
+    else                                                                                     \
+      launch_screen<OP, KV, 256>(grid, st, qpack, nq, rpack, nr, slice_len, S, pool, buf,    \
+                                 cand_keys, cand_cnt);                                       \
+    SH_LAUNCH_CHECK("k_screen");                                                             \
+    return SH_OK;                                                                            \
+  }
+  if (dtype == SH_BF16) {
+    SH_SCREEN_CASE(OpBF16, 16)
+    SH_SCREEN_CASE(OpBF16, 32)
+    SH_SCREEN_CASE(OpBF16, 48)
+    SH_SCREEN_CASE(OpBF16, 64)
+    SH_SCREEN_CASE(OpBF16, 80)
+    SH_SCREEN_CASE(OpBF16, 96)
+    SH_SCREEN_CASE(OpBF16, 112)
+    SH_SCREEN_CASE(OpBF16, 128)
+  } else if (dtype == SH_F32) {
+    SH_SCREEN_CASE(OpF32, 8)
+    SH_SCREEN_CASE(OpF32, 16)
+    SH_SCREEN_CASE(OpF32, 24)
+    SH_SCREEN_CASE(OpF32, 32)
+    SH_SCREEN_CASE(OpF32, 40)
+    SH_SCREEN_CASE(OpF32, 48)
+    SH_SCREEN_CASE(OpF32, 56)
+    SH_SCREEN_CASE(OpF32, 64)
+    SH_SCREEN_CASE(OpF32, 72)
+    SH_SCREEN_CASE(OpF32, 80)
+    SH_SCREEN_CASE(OpF32, 88)
+    SH_SCREEN_CASE(OpF32, 96)
+    SH_SCREEN_CASE(OpF32, 104)
+    SH_SCREEN_CASE(OpF32, 112)
+    SH_SCREEN_CASE(OpF32, 120)
+    SH_SCREEN_CASE(OpF32, 128)
+    SH_SCREEN_CASE(OpF32, 136)
+  }
+#undef SH_SCREEN_CASE
+  set_error("sh_knn_screen: dtype %d with packed K=%d not supported", dtype, K);
+  return SH_EUNSUPPORTED;
+}
+
+extern "C" int sh_knn_rerank(const float *Q, int64_t nq, int64_t ldq, const float *R, int64_t nr,
+                             int64_t ldr, int32_t D, const uint64_t *cand_keys,
+                             const int32_t *cand_cnt, int32_t pool, int32_t n_slices, int32_t k,
+                             int64_t self_offset, double err_coef, float r_norm_max,
+                             int64_t *idx_out, float *dist_out, int32_t *flags_out,
+                             sh_stream_t stream) {
+  SH_ARG(nq >= 0 && nr > 0 && D > 0 && D <= 256, "sh_knn_rerank: bad sizes");
+  SH_ARG(pool >= 1 && pool <= 128 && n_slices >= 1, "sh_knn_rerank: bad pool/slices");
+  const int kk = k + (self_offset >= 0 ? 1 : 0);
+  SH_ARG(k >= 1 && kk <= pool && kk <= 127, "sh_knn_rerank: k=%d (+self) must be <= pool=%d", k,
+         pool);
+  SH_ARG(Q && R && cand_keys && cand_cnt && idx_out && dist_out, "sh_knn_rerank: NULL pointer");
+  if (nq == 0) return SH_OK;
+  dim3 grid(cdiv(nq, 4)), block(256);
+  if (pool <= 64)
+    hipLaunchKernelGGL(k_rerank<1>, grid, block, 0, (hipStream_t)stream, Q, nq, ldq, R, nr, ldr,
+                       D, cand_keys, cand_cnt, pool, n_slices, k, self_offset, err_coef,
+                       r_norm_max, idx_out, dist_out, flags_out);
+  else
+    hipLaunchKernelGGL(k_rerank<2>, grid, block, 0, (hipStream_t)stream, Q, nq, ldq, R, nr, ldr,
+                       D, cand_keys, cand_cnt, pool, n_slices, k, self_offset, err_coef,
+                       r_norm_max, idx_out, dist_out, flags_out);
+  SH_LAUNCH_CHECK("k_rerank");
+  return SH_OK;
+}
+
+extern "C" int sh_knn_exact(const float *Q, int64_t nq, int64_t ldq, const int64_t *q_rows,
+                            int64_t n_sel, const float *R, int64_t nr, int64_t ldr, int32_t D,
+                            int32_t k, int64_t self_offset, int64_t *idx_out, float *dist_out,
+                            double *dist64_out, int32_t *flags_out, sh_stream_t stream) {
+  (void)nq;
+  SH_ARG(n_sel >= 0 && nr > 0 && D > 0 && D <= 256, "sh_knn_exact: bad sizes");
+  SH_ARG(nr < 0xffffffffll, "sh_knn_exact: more than 2^32-1 references");
+  const int kk = k + (self_offset >= 0 ? 1 : 0);
+  SH_ARG(k >= 1 && kk <= 64, "sh_knn_exact: k=%d (+self) must be <= 64", k);
+  SH_ARG(Q && R && idx_out && dist_out, "sh_knn_exact: NULL pointer");
+  if (n_sel == 0) return SH_OK;
+  dim3 grid(cdiv(n_sel, 4)), block(256);
+  hipStream_t st = (hipStream_t)stream;
+  if (kk <= 16)
+    hipLaunchKernelGGL(k_exact<16>, grid, block, 0, st, Q, ldq, q_rows, n_sel, R, nr, ldr, D, k,
+                       self_offset, idx_out, dist_out, dist64_out, flags_out);
+  else if (kk <= 32)
+    hipLaunchKernelGGL(k_exact<32>, grid, block, 0, st, Q, ldq, q_rows, n_sel, R, nr, ldr, D, k,
+                       self_offset, idx_out, dist_out, dist64_out, flags_out);
+  else
+    hipLaunchKernelGGL(k_exact<64>, grid, block, 0, st, Q, ldq, q_rows, n_sel, R, nr, ldr, D, k,
+                       self_offset, idx_out, dist_out, dist64_out, flags_out);
+  SH_LAUNCH_CHECK("k_exact");
+  return SH_OK;
+}

@@ -1,0 +1,91 @@
+// Mapping scores of reference cells (get_mapping_score, mapping_datastore.py:211-285):
+//   ms_g[ref] = multiplier / (n_g * k) * sum_{q in g, j} w(d_qj) [ref == idx_qj]
+//   w(d) = 1 / (log1p(d) + 1)  (or fixed_weight), then log1p(ms) if log_transform.
+// The sums are kept in unsigned 2^-40 fixed point and accumulated with 64-bit
+// integer atomics, which are associative: the result does not depend on launch
+// order or batch composition.
+#include "sh_common.h"
+
+namespace sh {
+
+constexpr double kFix = 1099511627776.0;  // 2^40
+
+__global__ void k_ms_clear(uint64_t *__restrict__ acc, int64_t m, int32_t *__restrict__ cnt,
+                           int32_t n_groups) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < m) acc[t] = 0;
+  if (t < n_groups) cnt[t] = 0;
+}
+
+__global__ void k_ms_scatter(const int64_t *__restrict__ idx, const float *__restrict__ dist,
+                             int64_t nq, int32_t k, const int32_t *__restrict__ groups,
+                             int32_t n_groups, int64_t n_ref, int weighted, double fixed_weight,
+                             uint64_t *__restrict__ acc, int32_t *__restrict__ cnt) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nq * k) return;
+  const int64_t q = t / k;
+  const int g = groups ? groups[q] : 0;
+  if (g < 0 || g >= n_groups) return;
+  if (t % k == 0) atomicAdd(&cnt[g], 1);
+  const int64_t r = idx[t];
+  if (r < 0 || r >= n_ref) return;
+  const double w = weighted ? 1.0 / (log1p((double)dist[t]) + 1.0) : fixed_weight;
+  const uint64_t f = (uint64_t)llrint(w * kFix);
+  atomicAdd((unsigned long long *)&acc[(int64_t)g * n_ref + r], (unsigned long long)f);
+}
+
+__global__ void k_ms_final(const uint64_t *__restrict__ acc, const int32_t *__restrict__ cnt,
+                           int32_t n_groups, int64_t n_ref, int32_t k, double multiplier,
+                           int log_t, double *__restrict__ out) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)n_groups * n_ref) return;
+  const int g = (int)(t / n_ref);
+  const double s = (double)acc[t] / kFix;
+  const double den = (double)cnt[g] * (double)k;
+  double ms = multiplier * s / den;
+  out[t] = log_t ? log1p(ms) : ms;
+}
+
+}  // namespace sh
+
+using namespace sh;
+
+extern "C" size_t sh_mapping_score_workspace(int64_t n_ref, int32_t n_groups) {
+  return align_up((size_t)n_ref * (n_groups > 0 ? n_groups : 1) * sizeof(uint64_t), 256) +
+         align_up((size_t)(n_groups > 0 ? n_groups : 1) * sizeof(int32_t), 256);
+}
+
+extern "C" int sh_mapping_score(const int64_t *idx, const float *dist, int64_t nq, int32_t k,
+                                const int32_t *groups, int32_t n_groups, int64_t n_ref,
+                                int weighted, double fixed_weight, double multiplier,
+                                int log_transform, double *out, void *workspace,
+                                size_t workspace_bytes, sh_stream_t stream) {
+  SH_ARG(nq >= 0 && k >= 1 && n_ref >= 1 && n_groups >= 1, "sh_mapping_score: bad sizes");
+  SH_ARG(groups || n_groups == 1, "sh_mapping_score: groups NULL requires n_groups == 1");
+  SH_ARG(idx && dist && out, "sh_mapping_score: NULL pointer");
+  SH_ARG(weighted || (fixed_weight > 0 && fixed_weight * kFix * nq * k < 1.8e19),
+         "sh_mapping_score: fixed_weight out of range");
+  SH_ARG((double)nq * k * kFix < 1.8e19, "sh_mapping_score: too many pairs for 2^-40 fixed point");
+  const size_t need = sh_mapping_score_workspace(n_ref, n_groups);
+  if (!workspace || workspace_bytes < need) {
+    set_error("sh_mapping_score: workspace %zu < %zu bytes", workspace_bytes, need);
+    return SH_EWORKSPACE;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  uint64_t *acc = (uint64_t *)workspace;
+  int32_t *cnt = (int32_t *)((char *)workspace +
+                             align_up((size_t)n_ref * n_groups * sizeof(uint64_t), 256));
+  const int64_t m = n_ref * n_groups;
+  hipLaunchKernelGGL(k_ms_clear, dim3(cdiv(m > n_groups ? m : n_groups, 256)), dim3(256), 0, st,
+                     acc, m, cnt, n_groups);
+  SH_LAUNCH_CHECK("k_ms_clear");
+  if (nq > 0) {
+    hipLaunchKernelGGL(k_ms_scatter, dim3(cdiv(nq * k, 256)), dim3(256), 0, st, idx, dist, nq, k,
+                       groups, n_groups, n_ref, weighted, fixed_weight, acc, cnt);
+    SH_LAUNCH_CHECK("k_ms_scatter");
+  }
+  hipLaunchKernelGGL(k_ms_final, dim3(cdiv(m, 256)), dim3(256), 0, st, acc, cnt, n_groups, n_ref,
+                     k, multiplier, log_transform, out);
+  SH_LAUNCH_CHECK("k_ms_final");
+  return SH_OK;
+}

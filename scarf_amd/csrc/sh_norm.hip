@@ -1,0 +1,380 @@
+// Library-size normalisation, per-feature statistics, dense z tiles and the sparse
+// z-score projection (HBM-bound CSR passes).
+//
+// Reference semantics (scarf tree): assay.py:41-64 (norm_lib_size[_log]),
+// assay.py:812-828 (renormalize_subset library size, 0 -> 1),
+// graph_datastore.py:767-796 + utils.py:143-154 (mean/std + clean_array),
+// ann.py:129-138 and 191-192 (z-score and reducer).
+#include <float.h>
+
+#include "sh_common.h"
+
+namespace sh {
+
+// x = log1p(sf * c / l) in FP64 with an IEEE division, as numpy evaluates it.
+__device__ __forceinline__ double norm_value(uint32_t c, double sf, double l, int log_t) {
+  double v = (sf * (double)c) / l;
+  return log_t ? log1p(v) : v;
+}
+
+__device__ __forceinline__ int map_col(const int32_t *col_map, int32_t c, int32_t n_in) {
+  if (c < 0 || c >= n_in) return -1;
+  return col_map ? col_map[c] : c;
+}
+
+// One wave per row: library size over the mapped columns; 0 -> 1.
+__global__ void __launch_bounds__(256) k_libsize(const uint32_t *__restrict__ vals,
+                                                 const int32_t *__restrict__ cols,
+                                                 const int64_t *__restrict__ rowptr,
+                                                 int64_t n_rows,
+                                                 const int32_t *__restrict__ col_map,
+                                                 int32_t n_in, double *__restrict__ out) {
+  const int lane = threadIdx.x & 63;
+  const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (row >= n_rows) return;
+  const int64_t a = rowptr[row], b = rowptr[row + 1];
+  uint64_t s = 0;
+  for (int64_t t = a + lane; t < b; t += 64) {
+    if (map_col(col_map, cols[t], n_in) >= 0) s += vals[t];
+  }
+  s = wave_sum_u64(s);
+  if (lane == 0) out[row] = s == 0 ? 1.0 : (double)s;
+}
+
+// Per-feature FP64 sums of x and x^2 over a slab of rows, for columns [c0, c0+cb).
+// Each wave owns a private LDS accumulator; within a row every mapped column is
+// distinct, so the read-modify-write needs no atomics and the order is fixed.
+constexpr int kStatsWaves = 2;
+__global__ void __launch_bounds__(64 * kStatsWaves) k_norm_stats(
+    const uint32_t *__restrict__ vals, const int32_t *__restrict__ cols,
+    const int64_t *__restrict__ rowptr, int64_t n_rows, int64_t rows_per_slab,
+    const int32_t *__restrict__ col_map, int32_t n_in, int32_t d, int32_t cb,
+    double sf, int log_t, const double *__restrict__ libsize,
+    double *__restrict__ part /* [n_slabs][2][d] */) {
+  extern __shared__ double acc[];  // [kStatsWaves][2][cb]
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int32_t c0 = blockIdx.y * cb;
+  const int32_t cend = min(d, c0 + cb);
+  const int32_t ncol = cend - c0;
+  double *my = acc + (size_t)wave * 2 * cb;
+  for (int i = threadIdx.x; i < kStatsWaves * 2 * cb; i += blockDim.x) acc[i] = 0.0;
+  __syncthreads();
+  const int64_t r0 = (int64_t)blockIdx.x * rows_per_slab;
+  const int64_t r1 = min(n_rows, r0 + rows_per_slab);
+  for (int64_t r = r0 + wave; r < r1; r += kStatsWaves) {
+    const int64_t a = rowptr[r], b = rowptr[r + 1];
+    const double l = libsize[r];
+    for (int64_t t = a + lane; t < b; t += 64) {
+      int f = map_col(col_map, cols[t], n_in);
+      if (f >= c0 && f < cend) {
+        double x = norm_value(vals[t], sf, l, log_t);
+        int j = f - c0;
+        my[j] += x;
+        my[cb + j] += x * x;
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+  }
+  __syncthreads();
+  double *dst = part + (size_t)blockIdx.x * 2 * d;
+  for (int j = threadIdx.x; j < ncol; j += blockDim.x) {
+    double s = 0.0, q = 0.0;
+#pragma unroll
+    for (int w = 0; w < kStatsWaves; ++w) {
+      s += acc[(size_t)w * 2 * cb + j];
+      q += acc[(size_t)w * 2 * cb + cb + j];
+    }
+    dst[c0 + j] = s;
+    dst[d + c0 + j] = q;
+  }
+}
+
+__global__ void k_reduce_slabs(const double *__restrict__ part, int n_slabs, int32_t d,
+                               double *__restrict__ sum_out, double *__restrict__ sq_out) {
+  int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= d) return;
+  double s = 0.0, q = 0.0;
+  for (int i = 0; i < n_slabs; ++i) {
+    s += part[(size_t)i * 2 * d + f];
+    q += part[(size_t)i * 2 * d + d + f];
+  }
+  sum_out[f] = s;
+  sq_out[f] = q;
+}
+
+// clean_array (utils.py:143-154): nan_to_num maps NaN -> 0 and +-inf -> +-DBL_MAX
+// first, so the later "+-inf -> 0" step never fires; then 0 -> fill.
+__device__ __forceinline__ double clean(double v, double fill) {
+  if (!(v == v)) v = 0.0;
+  else if (isinf(v)) v = v > 0 ? DBL_MAX : -DBL_MAX;
+  return v == 0.0 ? fill : v;
+}
+
+// Chan et al. pairwise merge of per-chunk (n, mean, M2), chunk order fixed.
+__global__ void k_stats_merge(const double *__restrict__ sums, const double *__restrict__ sqs,
+                              const int64_t *__restrict__ counts, int32_t n_chunks, int32_t d,
+                              double *__restrict__ mu, double *__restrict__ sigma,
+                              double *__restrict__ chunk_mean) {
+  int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= d) return;
+  double n = 0.0, mean = 0.0, m2 = 0.0;
+  for (int c = 0; c < n_chunks; ++c) {
+    double nc = (double)counts[c];
+    if (nc <= 0) {
+      if (chunk_mean) chunk_mean[(size_t)c * d + f] = 0.0;
+      continue;
+    }
+    double s = sums[(size_t)c * d + f], q = sqs[(size_t)c * d + f];
+    double mc = s / nc;
+    double m2c = q - s * mc;
+    if (m2c < 0) m2c = 0;
+    if (chunk_mean) chunk_mean[(size_t)c * d + f] = mc;
+    double nn = n + nc;
+    double delta = mc - mean;
+    mean += delta * (nc / nn);
+    m2 += m2c + delta * delta * (n * nc / nn);
+    n = nn;
+  }
+  double var = n > 0 ? m2 / n : 0.0;
+  mu[f] = clean(n > 0 ? mean : 0.0, 0.0);
+  sigma[f] = clean(sqrt(var), 1.0);
+}
+
+// One workgroup per row batch: the dense row is assembled in LDS, then streamed out.
+template <typename T>
+__global__ void __launch_bounds__(256) k_densify(
+    const uint32_t *__restrict__ vals, const int32_t *__restrict__ cols,
+    const int64_t *__restrict__ rowptr, const int64_t *__restrict__ rows, int64_t n_rows,
+    const int32_t *__restrict__ col_map, int32_t n_in, int32_t d, double sf, int log_t,
+    const double *__restrict__ libsize, const double *__restrict__ center,
+    const double *__restrict__ scale, double clip, T *__restrict__ out, int64_t ld,
+    int rows_per_block) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char dbuf_raw[];
+  T *buf = reinterpret_cast<T *>(dbuf_raw);  // [ld]
+  const int64_t rb = (int64_t)blockIdx.x * rows_per_block;
+  for (int rr = 0; rr < rows_per_block; ++rr) {
+    const int64_t i = rb + rr;
+    if (i >= n_rows) break;
+    const int64_t r = rows ? rows[i] : i;
+    for (int64_t f = threadIdx.x; f < ld; f += blockDim.x) {
+      double v = 0.0;
+      if (f < d) {
+        v = (0.0 - center[f]) * scale[f];
+        if (clip > 0.0) v = fmin(fmax(v, -clip), clip);
+      }
+      buf[f] = (T)v;
+    }
+    __syncthreads();
+    const int64_t a = rowptr[r], b = rowptr[r + 1];
+    const double l = libsize[r];
+    for (int64_t t = a + threadIdx.x; t < b; t += blockDim.x) {
+      int f = map_col(col_map, cols[t], n_in);
+      if (f >= 0) {
+        double x = norm_value(vals[t], sf, l, log_t);
+        double v = (x - center[f]) * scale[f];
+        if (clip > 0.0) v = fmin(fmax(v, -clip), clip);
+        buf[f] = (T)v;
+      }
+    }
+    __syncthreads();
+    float4 *o4 = reinterpret_cast<float4 *>(out + i * ld);
+    const float4 *b4 = reinterpret_cast<const float4 *>(buf);
+    const int64_t n16 = ld * (int64_t)sizeof(T) / 16;
+    for (int64_t f = threadIdx.x; f < n16; f += blockDim.x) o4[f] = b4[f];
+    __syncthreads();
+  }
+}
+
+// Sparse projection: one wave per row; lane j accumulates output dims j and j+64.
+// The row's (feature, z-delta) pairs are read 64 at a time and broadcast with
+// readlane, so each non-zero costs one coalesced load of a W row per lane.
+template <int DPL>
+__global__ void __launch_bounds__(256) k_project(
+    const uint32_t *__restrict__ vals, const int32_t *__restrict__ cols,
+    const int64_t *__restrict__ rowptr, int64_t n_rows, const int32_t *__restrict__ col_map,
+    int32_t n_in, double sf, int log_t, const double *__restrict__ libsize,
+    const float *__restrict__ mu, const float *__restrict__ inv_sigma, float clip,
+    const float *__restrict__ W, int64_t ldw, int32_t D, const float *__restrict__ cvec,
+    float *__restrict__ E, int64_t lde) {
+  const int lane = threadIdx.x & 63;
+  const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (row >= n_rows) return;
+  const int64_t a = rowptr[row], b = rowptr[row + 1];
+  double l;
+  if (libsize) {
+    l = libsize[row];
+  } else {
+    uint64_t s = 0;
+    for (int64_t t = a + lane; t < b; t += 64)
+      if (map_col(col_map, cols[t], n_in) >= 0) s += vals[t];
+    s = wave_sum_u64(s);
+    l = s == 0 ? 1.0 : (double)s;
+  }
+  float acc[DPL];
+#pragma unroll
+  for (int p = 0; p < DPL; ++p) acc[p] = 0.f;
+  for (int64_t t0 = a; t0 < b; t0 += 64) {
+    const int64_t t = t0 + lane;
+    int f = -1;
+    float delta = 0.f;
+    if (t < b) {
+      f = map_col(col_map, cols[t], n_in);
+      if (f >= 0) {
+        float x = (float)norm_value(vals[t], sf, l, log_t);
+        if (clip > 0.f) {
+          float zs = (x - mu[f]) * inv_sigma[f];
+          float z0 = (0.f - mu[f]) * inv_sigma[f];
+          delta = fminf(fmaxf(zs, -clip), clip) - fminf(fmaxf(z0, -clip), clip);
+        } else {
+          delta = x * inv_sigma[f];
+        }
+      }
+    }
+    const int cnt = (int)min<int64_t>(64, b - t0);
+#pragma unroll 8
+    for (int j = 0; j < cnt; ++j) {
+      int fj = __builtin_amdgcn_readlane(f, j);
+      float dj = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(delta), j));
+      if (fj >= 0) {
+        const float *wr = W + (int64_t)fj * ldw;
+#pragma unroll
+        for (int p = 0; p < DPL; ++p) {
+          int c = lane + 64 * p;
+          if (c < D) acc[p] = fmaf(dj, wr[c], acc[p]);
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int p = 0; p < DPL; ++p) {
+    int c = lane + 64 * p;
+    if (c < D) E[row * lde + c] = acc[p] + cvec[c];
+  }
+}
+
+}  // namespace sh
+
+using namespace sh;
+
+static int64_t stats_slabs(int64_t n_rows) {
+  int64_t s = n_rows / 64;
+  if (s < 1) s = 1;
+  if (s > 1024) s = 1024;
+  return s;
+}
+
+extern "C" size_t sh_norm_stats_workspace(int64_t n_rows, int32_t d) {
+  return (size_t)stats_slabs(n_rows) * 2 * (size_t)(d > 0 ? d : 1) * sizeof(double);
+}
+
+extern "C" int sh_norm_stats(const uint32_t *vals, const int32_t *cols, const int64_t *rowptr,
+                             int64_t n_rows, const int32_t *col_map, int32_t n_cols_in,
+                             int32_t d, double sf, int log_transform,
+                             const double *libsize_in, double *libsize_out, double *sum_out,
+                             double *sumsq_out, void *workspace, size_t workspace_bytes,
+                             sh_stream_t stream) {
+  hipStream_t st = (hipStream_t)stream;
+  SH_ARG(n_rows >= 0 && d > 0 && n_cols_in > 0, "sh_norm_stats: bad sizes n_rows=%lld d=%d n_cols_in=%d",
+         (long long)n_rows, d, n_cols_in);
+  SH_ARG(col_map || d == n_cols_in, "sh_norm_stats: col_map NULL requires d == n_cols_in");
+  SH_ARG(rowptr && sum_out && sumsq_out, "sh_norm_stats: NULL pointer");
+  SH_ARG(sf > 0, "sh_norm_stats: size factor must be > 0");
+  if (n_rows == 0) {
+    SH_HIP(hipMemsetAsync(sum_out, 0, sizeof(double) * d, st));
+    SH_HIP(hipMemsetAsync(sumsq_out, 0, sizeof(double) * d, st));
+    return SH_OK;
+  }
+  const size_t need = sh_norm_stats_workspace(n_rows, d);
+  if (workspace_bytes < need || !workspace) {
+    set_error("sh_norm_stats: workspace %zu < %zu bytes", workspace_bytes, need);
+    return SH_EWORKSPACE;
+  }
+  const double *ls = libsize_in;
+  if (!ls) {
+    SH_ARG(libsize_out, "sh_norm_stats: libsize_out required when libsize_in is NULL");
+    hipLaunchKernelGGL(k_libsize, dim3(cdiv(n_rows, 4)), dim3(256), 0, st, vals, cols, rowptr,
+                       n_rows, col_map, n_cols_in, libsize_out);
+    SH_LAUNCH_CHECK("k_libsize");
+    ls = libsize_out;
+  } else if (libsize_out && libsize_out != libsize_in) {
+    SH_HIP(hipMemcpyAsync(libsize_out, libsize_in, sizeof(double) * n_rows,
+                          hipMemcpyDeviceToDevice, st));
+  }
+  const int64_t slabs = stats_slabs(n_rows);
+  const int64_t rps = cdiv(n_rows, slabs);
+  const int32_t cb = d < 2048 ? d : 2048;  // 2 waves x 2 x 2048 doubles = 64 KiB
+  const int n_cb = (int)cdiv(d, cb);
+  const size_t lds = (size_t)kStatsWaves * 2 * cb * sizeof(double);
+  double *part = (double *)workspace;
+  hipLaunchKernelGGL(k_norm_stats, dim3((unsigned)slabs, n_cb), dim3(64 * kStatsWaves), lds, st,
+                     vals, cols, rowptr, n_rows, rps, col_map, n_cols_in, d, cb, sf,
+                     log_transform, ls, part);
+  SH_LAUNCH_CHECK("k_norm_stats");
+  hipLaunchKernelGGL(k_reduce_slabs, dim3(cdiv(d, 256)), dim3(256), 0, st, part, (int)slabs, d,
+                     sum_out, sumsq_out);
+  SH_LAUNCH_CHECK("k_reduce_slabs");
+  return SH_OK;
+}
+
+extern "C" int sh_stats_merge(const double *sums, const double *sumsqs, const int64_t *counts,
+                              int32_t n_chunks, int32_t d, double *mu_out, double *sigma_out,
+                              double *chunk_mean_out, sh_stream_t stream) {
+  SH_ARG(n_chunks > 0 && d > 0, "sh_stats_merge: bad sizes");
+  SH_ARG(sums && sumsqs && counts && mu_out && sigma_out, "sh_stats_merge: NULL pointer");
+  hipLaunchKernelGGL(k_stats_merge, dim3(cdiv(d, 256)), dim3(256), 0, (hipStream_t)stream, sums,
+                     sumsqs, counts, n_chunks, d, mu_out, sigma_out, chunk_mean_out);
+  SH_LAUNCH_CHECK("k_stats_merge");
+  return SH_OK;
+}
+
+extern "C" int sh_densify(const uint32_t *vals, const int32_t *cols, const int64_t *rowptr,
+                          const int64_t *rows, int64_t n_rows, const int32_t *col_map,
+                          int32_t n_cols_in, int32_t d, double sf, int log_transform,
+                          const double *libsize, const double *center, const double *scale,
+                          double clip, int out_f64, void *out, int64_t ld, sh_stream_t stream) {
+  SH_ARG(n_rows >= 0 && d > 0 && ld >= d && ld % 4 == 0, "sh_densify: bad sizes d=%d ld=%lld", d,
+         (long long)ld);
+  SH_ARG(col_map || d == n_cols_in, "sh_densify: col_map NULL requires d == n_cols_in");
+  SH_ARG(libsize && center && scale && out && rowptr, "sh_densify: NULL pointer");
+  const size_t esz = out_f64 ? sizeof(double) : sizeof(float);
+  SH_ARG(ld * esz <= 64 * 1024, "sh_densify: ld=%lld exceeds the 64 KiB row buffer",
+         (long long)ld);
+  if (n_rows == 0) return SH_OK;
+  const int rpb = 2;
+  if (out_f64)
+    hipLaunchKernelGGL(k_densify<double>, dim3(cdiv(n_rows, rpb)), dim3(256), ld * esz,
+                       (hipStream_t)stream, vals, cols, rowptr, rows, n_rows, col_map, n_cols_in,
+                       d, sf, log_transform, libsize, center, scale, clip, (double *)out, ld, rpb);
+  else
+    hipLaunchKernelGGL(k_densify<float>, dim3(cdiv(n_rows, rpb)), dim3(256), ld * esz,
+                       (hipStream_t)stream, vals, cols, rowptr, rows, n_rows, col_map, n_cols_in,
+                       d, sf, log_transform, libsize, center, scale, clip, (float *)out, ld, rpb);
+  SH_LAUNCH_CHECK("k_densify");
+  return SH_OK;
+}
+
+extern "C" int sh_project(const uint32_t *vals, const int32_t *cols, const int64_t *rowptr,
+                          int64_t n_rows, const int32_t *col_map, int32_t n_cols_in, int32_t d,
+                          double sf, int log_transform, const double *libsize, const float *mu,
+                          const float *inv_sigma, float clip, const float *W, int64_t ldw,
+                          int32_t D, const float *c, float *E_out, int64_t lde,
+                          sh_stream_t stream) {
+  SH_ARG(n_rows >= 0 && d > 0 && D > 0 && D <= 128 && ldw >= D && lde >= D,
+         "sh_project: bad sizes d=%d D=%d ldw=%lld lde=%lld", d, D, (long long)ldw,
+         (long long)lde);
+  SH_ARG(col_map || d == n_cols_in, "sh_project: col_map NULL requires d == n_cols_in");
+  SH_ARG(mu && inv_sigma && W && c && E_out && rowptr, "sh_project: NULL pointer");
+  if (n_rows == 0) return SH_OK;
+  dim3 grid(cdiv(n_rows, 4)), block(256);
+  if (D <= 64)
+    hipLaunchKernelGGL(k_project<1>, grid, block, 0, (hipStream_t)stream, vals, cols, rowptr,
+                       n_rows, col_map, n_cols_in, sf, log_transform, libsize, mu, inv_sigma,
+                       clip, W, ldw, D, c, E_out, lde);
+  else
+    hipLaunchKernelGGL(k_project<2>, grid, block, 0, (hipStream_t)stream, vals, cols, rowptr,
+                       n_rows, col_map, n_cols_in, sf, log_transform, libsize, mu, inv_sigma,
+                       clip, W, ldw, D, c, E_out, lde);
+  SH_LAUNCH_CHECK("k_project");
+  return SH_OK;
+}

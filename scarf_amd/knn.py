@@ -1,0 +1,294 @@
+"""Exact kNN on the GPU behind an hnswlib-compatible index object.
+
+scarf builds an ``hnswlib.Index(space='l2')`` (``ann.py:14-28``), fills it block by
+block with ``add_items`` (``ann.py:322-325``) and queries it with ``knn_query``
+(``ann.py:194-205``), which returns uint64 labels and float32 *squared* Euclidean
+distances in ascending order.  ``GpuIndex`` keeps the reference embedding resident in
+HBM and answers ``knn_query`` with an exact search:
+
+* ``screen="fp32"``: FP32 MFMA screen, exact FP64 re-rank of the pool, and an
+  error-bound certificate; rows it cannot certify are recomputed by FP64 brute
+  force.  The result equals the FP64 brute-force top-k (ties to the lower index).
+* ``screen="bf16"``: BF16 MFMA screen (FP32 accumulation) keeping a ``pool`` of
+  candidates per query, re-ranked exactly; recall against FP64 is measured.
+
+Self-queries (``self_offset``) return k neighbours after scarf's ``fix_knn_query``
+repair (``ann.py:31-52``).
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+import os
+from typing import Optional
+
+import numpy as np
+
+from . import _lib
+from .device import ptr, require_gpu, stream_handle, workspace
+
+try:
+    import torch
+except ImportError:  # pragma: no cover
+    torch = None
+
+U32 = 2.0 ** -24
+
+
+@dataclasses.dataclass
+class PackedRefs:
+    dtype: str
+    D: int
+    K: int
+    data: "torch.Tensor"  # packed operand rows
+    norms: "torch.Tensor"  # f32 ||r||^2
+    norm_max: float
+
+
+def pack(E: "torch.Tensor", dtype: str, role: int, stream=None):
+    """Pack an FP32 embedding [n, D] into the screen operand layout."""
+    require_gpu()
+    code = _lib.DTYPES[dtype]
+    n, D = E.shape
+    K = int(_lib.lib().sh_knn_pack_k(code, D))
+    if K <= 0:
+        raise ValueError(f"screen dtype {dtype} unsupported for D={D}")
+    tdt = {"fp32": torch.float32, "f32": torch.float32, "bf16": torch.int16}[dtype]
+    out = torch.empty((max(n, 1), K), dtype=tdt, device=E.device)
+    aux = torch.empty(max(n, 1), dtype=torch.float32, device=E.device)
+    _lib.call("sh_knn_pack", ptr(E), n, D, E.stride(0), code, role, ptr(out), ptr(aux),
+              stream_handle(stream))
+    return out[:n], aux[:n], K
+
+
+def pack_refs(E: "torch.Tensor", dtype: str, stream=None) -> PackedRefs:
+    data, norms, K = pack(E, dtype, 0, stream)
+    nm = float(torch.sqrt(norms.max()).item()) if norms.numel() else 0.0
+    return PackedRefs(dtype, int(E.shape[1]), K, data, norms, nm)
+
+
+def default_slices(nq: int, nr: int) -> int:
+    """Reference slices so that small query batches still fill the GPU (>= ~512 WGs)."""
+    qblocks = max(1, math.ceil(nq / 256))
+    want = max(1, math.ceil(512 / qblocks))
+    max_by_len = max(1, nr // 2048)  # keep slices >= 2048 references
+    return int(min(want, max_by_len, 4096))
+
+
+def _fix_slices(nr: int, s: int) -> int:
+    # the kernel requires every slice to hold >= 1 reference after 64-alignment
+    while s > 1:
+        sl = int(math.ceil(math.ceil(nr / s) / 64) * 64)
+        if math.ceil(nr / sl) == s:
+            return s
+        s -= 1
+    return 1
+
+
+def search(Q: "torch.Tensor", R: "torch.Tensor", refs: PackedRefs, k: int, pool: int = 64,
+           self_offset: Optional[int] = None, n_slices: Optional[int] = None,
+           certify: Optional[bool] = None, stream=None):
+    """Top-k of each query row of ``Q`` against ``R`` (both FP32 [*, D] on device).
+
+    Returns (idx int64 [nq, k], dist f32 [nq, k], flags int32 [nq]).
+    """
+    require_gpu()
+    nq, D = Q.shape
+    nr = R.shape[0]
+    dev = Q.device
+    if D != refs.D:
+        raise ValueError(f"query dim {D} != index dim {refs.D}")
+    kk = k + (1 if self_offset is not None else 0)
+    if kk > nr:
+        raise ValueError(f"k={k} needs at least {kk} reference rows, index has {nr}")
+    pool = max(int(pool), kk)
+    if pool > 128:
+        raise ValueError("pool > 128 is not supported")
+    idx = torch.empty((nq, k), dtype=torch.int64, device=dev)
+    dist = torch.empty((nq, k), dtype=torch.float32, device=dev)
+    flags = torch.zeros(max(nq, 1), dtype=torch.int32, device=dev)
+    if nq == 0:
+        return idx, dist, flags[:0]
+    code = _lib.DTYPES[refs.dtype]
+    st = stream_handle(stream)
+    qp, _, _ = pack(Q, refs.dtype, 1, stream)
+    S = _fix_slices(nr, n_slices if n_slices else default_slices(nq, nr))
+    cand = torch.empty((nq, S, pool), dtype=torch.int64, device=dev)
+    cnt = torch.empty((nq, S), dtype=torch.int32, device=dev)
+    ws, nbytes = workspace("screen").get(_lib.lib().sh_knn_screen_workspace(nq, pool, S))
+    _lib.call("sh_knn_screen", ptr(qp), nq, ptr(refs.data), nr, D, code, None, None, pool, S,
+              ptr(cand), ptr(cnt), ptr(ws), nbytes, st)
+    if certify is None:
+        certify = refs.dtype in ("fp32", "f32")
+    err_coef = -1.0
+    if certify:
+        # |s~ - s| <= (K + 2) u (||r||^2 + 2 |q| |r|) for the FP32 screen; x2 margin
+        err_coef = 2.0 * (refs.K + 2) * U32
+    so = -1 if self_offset is None else int(self_offset)
+    _lib.call("sh_knn_rerank", ptr(Q), nq, Q.stride(0), ptr(R), nr, R.stride(0), D, ptr(cand),
+              ptr(cnt), pool, S, k, so, float(err_coef), float(refs.norm_max), ptr(idx),
+              ptr(dist), ptr(flags), st)
+    if certify:
+        bad = torch.nonzero(flags[:nq] & _lib.SH_FLAG_UNCERTIFIED).flatten()
+        if bad.numel():
+            exact_rows(Q, R, k, bad, idx, dist, flags, self_offset, stream)
+    return idx, dist, flags[:nq]
+
+
+def exact_rows(Q, R, k, rows, idx, dist, flags, self_offset=None, stream=None, dist64=None):
+    """FP64 brute force for the query rows listed in ``rows`` (device int64)."""
+    nq, D = Q.shape
+    so = -1 if self_offset is None else int(self_offset)
+    _lib.call("sh_knn_exact", ptr(Q), nq, Q.stride(0), ptr(rows), int(rows.numel()), ptr(R),
+              R.shape[0], R.stride(0), D, k, so, ptr(idx), ptr(dist), ptr(dist64), ptr(flags),
+              stream_handle(stream))
+
+
+def exact_knn(Q, R, k, self_offset=None, stream=None):
+    """FP64 brute force for every query row (the GPU reference for recall)."""
+    require_gpu()
+    nq = Q.shape[0]
+    dev = Q.device
+    idx = torch.empty((nq, k), dtype=torch.int64, device=dev)
+    dist = torch.empty((nq, k), dtype=torch.float32, device=dev)
+    d64 = torch.empty((nq, k), dtype=torch.float64, device=dev)
+    flags = torch.zeros(max(nq, 1), dtype=torch.int32, device=dev)
+    so = -1 if self_offset is None else int(self_offset)
+    _lib.call("sh_knn_exact", ptr(Q), nq, Q.stride(0), None, nq, ptr(R), R.shape[0],
+              R.stride(0), Q.shape[1], k, so, ptr(idx), ptr(dist), ptr(d64), ptr(flags),
+              stream_handle(stream))
+    return idx, dist, d64, flags[:nq]
+
+
+class GpuIndex:
+    """hnswlib ``Index`` duck type (space 'l2') with an exact GPU search.
+
+    Constructor ``(space, dim)``; ``init_index``, ``add_items``, ``knn_query``,
+    ``set_ef``, ``set_num_threads``, ``save_index``, ``load_index``,
+    ``get_current_count`` behave as scarf uses them (``ann.py:14-28``,
+    ``graph_datastore.py:875-881, 945-947``).  ``ef``/``M``/threads are accepted and
+    recorded; they do not change the (exact) result.
+    """
+
+    def __init__(self, space: str = "l2", dim: int = 0, screen: str = "bf16", pool: int = 64,
+                 device: str = "cuda"):
+        if space != "l2":
+            raise ValueError(f"only space='l2' is supported (got {space!r})")
+        self.space = space
+        self.dim = int(dim)
+        self.screen = screen
+        self.pool = int(pool)
+        self.device = device
+        self.max_elements = 0
+        self.ef = 10
+        self.ef_construction = 200
+        self.M = 16
+        self.random_seed = 100
+        self.num_threads = 1
+        self._chunks = []
+        self._E = None
+        self._packed = None
+
+    # -- construction -----------------------------------------------------------
+    def init_index(self, max_elements: int, ef_construction: int = 200, M: int = 16,
+                   random_seed: int = 100, allow_replace_deleted: bool = False):
+        self.max_elements = int(max_elements)
+        self.ef_construction = int(ef_construction)
+        self.M = int(M)
+        self.random_seed = int(random_seed)
+
+    def set_ef(self, ef: int):
+        self.ef = int(ef)
+
+    def set_num_threads(self, n: int):
+        self.num_threads = int(n)
+
+    def _as_f32(self, data):
+        if torch is not None and isinstance(data, torch.Tensor):
+            t = data.to(device=self.device, dtype=torch.float32)
+        else:
+            a = np.ascontiguousarray(np.asarray(data), dtype=np.float32)
+            if a.ndim == 1:
+                a = a[None, :]
+            t = torch.from_numpy(a).to(self.device)
+        if t.ndim == 1:
+            t = t[None, :]
+        if t.shape[1] != self.dim:
+            raise RuntimeError(f"Wrong dimensionality of the vectors: {t.shape[1]} != {self.dim}")
+        return t.contiguous()
+
+    def add_items(self, data, ids=None, num_threads: int = -1, replace_deleted=False):
+        require_gpu()
+        if ids is not None:
+            ids = np.asarray(ids).ravel()
+            start = self.get_current_count()
+            if not np.array_equal(ids, np.arange(start, start + ids.shape[0])):
+                raise ValueError("GpuIndex labels are the insertion order; custom ids unsupported")
+        t = self._as_f32(data)
+        if self.max_elements and self.get_current_count() + t.shape[0] > self.max_elements:
+            raise RuntimeError("The number of elements exceeds the specified limit")
+        self._chunks.append(t)
+        self._E = None
+        self._packed = None
+
+    def add_items_tensor(self, E: "torch.Tensor"):
+        """Adds rows already resident on the device without a host round trip."""
+        self.add_items(E)
+
+    def get_current_count(self) -> int:
+        if self._E is not None:
+            return int(self._E.shape[0])
+        return int(sum(c.shape[0] for c in self._chunks))
+
+    @property
+    def element_count(self) -> int:
+        return self.get_current_count()
+
+    def embedding(self) -> "torch.Tensor":
+        if self._E is None:
+            if not self._chunks:
+                return torch.empty((0, self.dim), dtype=torch.float32, device=self.device)
+            self._E = torch.cat(self._chunks) if len(self._chunks) > 1 else self._chunks[0]
+            self._chunks = [self._E]
+        return self._E
+
+    def packed(self) -> PackedRefs:
+        if self._packed is None:
+            self._packed = pack_refs(self.embedding(), self.screen)
+        return self._packed
+
+    # -- query --------------------------------------------------------------------
+    def knn_query_tensor(self, Q: "torch.Tensor", k: int = 1, self_offset=None):
+        """Device in, device out: (int64 idx, f32 squared distances, flags)."""
+        return search(Q, self.embedding(), self.packed(), k, self.pool, self_offset)
+
+    def knn_query(self, data, k: int = 1, num_threads: int = -1, filter=None):
+        require_gpu()
+        n = self.get_current_count()
+        if k > n:
+            raise RuntimeError("Cannot return the results in a contiguous 2D array. "
+                               "Probably ef or M is too small")
+        Q = self._as_f32(data)
+        idx, dist, _ = self.knn_query_tensor(Q, k)
+        return idx.cpu().numpy().astype(np.uint64), dist.cpu().numpy()
+
+    # -- persistence ----------------------------------------------------------------
+    def save_index(self, path: str):
+        E = self.embedding().cpu().numpy()
+        meta = np.array([self.dim, self.max_elements, self.ef, self.ef_construction, self.M,
+                         self.random_seed, self.pool], dtype=np.int64)
+        with open(path, "wb") as f:
+            np.savez(f, embedding=E, meta=meta, screen=np.array(self.screen))
+
+    def load_index(self, path: str, max_elements: int = 0, allow_replace_deleted=False):
+        with np.load(path, allow_pickle=False) as z:
+            E = z["embedding"]
+            meta = z["meta"]
+            self.screen = str(z["screen"])
+        self.dim = int(meta[0])
+        self.max_elements = max(int(meta[1]), int(max_elements))
+        self.ef, self.ef_construction, self.M, self.random_seed, self.pool = \
+            (int(x) for x in meta[2:7])
+        self._chunks = [torch.from_numpy(np.ascontiguousarray(E, dtype=np.float32)).to(self.device)]
+        self._E = None
+        self._packed = None

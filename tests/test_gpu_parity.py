@@ -1,0 +1,266 @@
+"""GPU parity: every HIP stage against the CPU oracle on the same seeded inputs."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import ipca as oipca
+from oracle import knn as oknn
+from oracle import mapping as omap
+from oracle import norm as onorm
+from oracle import smooth as osmooth
+from scarf_amd.synth import CSR, generate_csr, make_gene_model
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def small_counts(n=3000, d=300, groups=4, seed=21, zero_rows=60):
+    model = make_gene_model(d, groups, 0.1, seed)
+    csr = generate_csr(n, model, seed)
+    dense = csr.to_dense().astype(np.uint32)
+    rng = np.random.default_rng(seed)
+    dense[rng.choice(n, zero_rows, replace=False)] = 0
+    return CSR.from_dense(dense), dense
+
+
+@pytest.fixture(scope="module")
+def data():
+    return small_counts()
+
+
+def test_norm_stats_parity(cuda, data):
+    import torch
+    from scarf_amd.device import DeviceCSR, col_map_tensor
+    from scarf_amd.normalize import NormParams, feature_stats
+    csr, dense = data
+    feat = np.sort(np.random.default_rng(0).choice(dense.shape[1], 220, replace=False))
+    for sf, chunk in ((1000.0, 700), (10000.0, 3000)):
+        say("norm_stats sf", sf, "chunk", chunk)
+        dev = DeviceCSR.from_host(csr)
+        cm, d = col_map_tensor(csr.n_cols, feat)
+        st = feature_stats(dev, chunk, NormParams(size_factor=sf), col_map=cm, d=d)
+        torch.cuda.synchronize()
+        x = onorm.normed(dense.astype(np.float64), sf=sf, feat_idx=feat)
+        mu, sigma = onorm.mu_sigma(x)
+        ls = onorm.libsize(dense.astype(np.float64), feat)
+        np.testing.assert_array_equal(st.libsize.cpu().numpy(), ls)
+        np.testing.assert_allclose(st.mu.cpu().numpy(), mu, rtol=1e-12, atol=1e-15)
+        np.testing.assert_allclose(st.sigma.cpu().numpy(), sigma, rtol=1e-12)
+
+
+def test_norm_stats_deterministic(cuda, data):
+    from scarf_amd.device import DeviceCSR
+    from scarf_amd.normalize import NormParams, feature_stats
+    csr, _ = data
+    dev = DeviceCSR.from_host(csr)
+    a = feature_stats(dev, 500, NormParams())
+    b = feature_stats(dev, 500, NormParams())
+    assert bool((a.mu == b.mu).all()) and bool((a.sigma == b.sigma).all())
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_gram_parity(cuda, data, precision):
+    import torch
+    from scarf_amd.device import DeviceCSR
+    from scarf_amd.normalize import NormParams, feature_stats
+    from scarf_amd.pca import gram_of_rows
+    csr, dense = data
+    dev = DeviceCSR.from_host(csr)
+    p = NormParams()
+    st = feature_stats(dev, 1000, p)
+    rows = torch.arange(100, 2900, device="cuda", dtype=torch.int64)
+    say("gram rows 100..2900 d", csr.n_cols, precision)
+    G, cs, n = gram_of_rows(dev, st, p, rows, sub_rows=1000, precision=precision)
+    x = onorm.normed(dense.astype(np.float64), sf=1000.0)
+    mu, sigma = onorm.mu_sigma(x)
+    z = ((x - mu) / sigma)[100:2900]
+    Gr = z.T @ z
+    # elementwise against the summation bound  c * u * sum_r |z_ri z_rj|
+    scale = np.abs(z).T @ np.abs(z)
+    err = (np.abs(G - Gr) / scale).max()
+    say("gram max error / sum|z z|", err)
+    assert err < (1e-13 if precision == "fp64" else 2e-5)
+    assert n == 2800
+    np.testing.assert_allclose(cs, z.sum(0), atol=1e-9 if precision == "fp64" else 2e-3)
+
+
+@pytest.mark.parametrize("name", ["ipca_small", "ipca_blocks"])
+def test_ipca_golden_parity(cuda, name):
+    """Loadings and PCA scores vs scikit-learn IncrementalPCA in scarf's order."""
+    from scarf_amd.device import DeviceCSR
+    from scarf_amd.normalize import NormParams, feature_stats
+    from scarf_amd.pca import Projector, fit_pca
+    g = np.load(os.path.join(GOLD, name + ".npz"))
+    csr = CSR(g["vals"], g["cols"], g["rowptr"], int(g["n_cols"]))
+    p = NormParams(size_factor=float(g["sf"]))
+    dev = DeviceCSR.from_host(csr)
+    chunk, dims = int(g["chunk"]), int(g["dims"])
+    say("ipca golden", name)
+    st = feature_stats(dev, chunk, p)
+    np.testing.assert_allclose(st.mu.cpu().numpy(), g["mu"], rtol=1e-12, atol=1e-15)
+    np.testing.assert_allclose(st.sigma.cpu().numpy(), g["sigma"], rtol=1e-12)
+    fit = fit_pca(dev, st, p, dims, chunk)
+    L = fit.loadings
+    # scores: E = z @ loadings; compare per component after sign alignment
+    proj = Projector.build(L, st.mu.cpu().numpy(), st.sigma.cpu().numpy())
+    E = proj.project(dev, p, libsize=st.libsize).cpu().numpy().astype(np.float64)
+    Er = g["embedding"]
+    for j in range(dims):
+        s = np.sign(np.dot(E[:, j], Er[:, j])) or 1.0
+        rel = np.linalg.norm(s * E[:, j] - Er[:, j]) / np.linalg.norm(Er[:, j])
+        assert rel < 1e-4, (name, j, rel)
+    np.testing.assert_allclose(fit.state.explained_variance_ratio_, g["explained_variance_ratio"],
+                               rtol=1e-4)
+
+
+def test_project_parity(cuda, data):
+    from scarf_amd.device import DeviceCSR
+    from scarf_amd.normalize import NormParams, feature_stats
+    from scarf_amd.pca import Projector
+    csr, dense = data
+    dev = DeviceCSR.from_host(csr)
+    rng = np.random.default_rng(4)
+    for D, clip in ((50, None), (100, 10.0), (13, None)):
+        say("project D", D, "clip", clip)
+        p = NormParams(clip=clip)
+        st = feature_stats(dev, 1000, p)
+        W = rng.standard_normal((csr.n_cols, D)) / np.sqrt(csr.n_cols)
+        proj = Projector.build(W, st.mu.cpu().numpy(), st.sigma.cpu().numpy(), clip=clip)
+        E = proj.project(dev, p, libsize=st.libsize).cpu().numpy()
+        x = onorm.normed(dense.astype(np.float64), sf=1000.0)
+        mu, sigma = onorm.mu_sigma(x)
+        z = onorm.transform_z_clipped(x, mu, sigma, clip)
+        Er = z @ W
+        err = np.abs(E - Er).max() / np.abs(Er).max()
+        say("project rel err", err)
+        assert err < 1e-5
+
+
+def _rand_embedding(n, D, seed, dup_rows=0):
+    rng = np.random.default_rng(seed)
+    c = rng.standard_normal((8, D)) * 3
+    E = (c[rng.integers(0, 8, n)] + rng.standard_normal((n, D))).astype(np.float32)
+    if dup_rows:
+        E[rng.choice(n, dup_rows, replace=False)] = E[0]
+    return E
+
+
+@pytest.mark.parametrize("nq,nr,D,k,dup", [(700, 3000, 50, 11, 0), (300, 5000, 50, 21, 40),
+                                           (50, 2000, 100, 11, 0), (1, 700, 13, 5, 0)])
+def test_knn_fp32_exact_bitwise(cuda, nq, nr, D, k, dup):
+    import torch
+    from scarf_amd.knn import pack_refs, search
+    R = _rand_embedding(nr, D, 5, dup)
+    Q = np.concatenate([R[:nq // 2], _rand_embedding(nq - nq // 2, D, 6)])
+    say("knn fp32 exact", nq, nr, D, k, dup)
+    Rt, Qt = torch.from_numpy(R).cuda(), torch.from_numpy(Q).cuda()
+    refs = pack_refs(Rt, "fp32")
+    idx, dist, flags = search(Qt, Rt, refs, k, pool=max(32, k + 1))
+    ri, rd = oknn.knn_f64(Q, R, k)
+    np.testing.assert_array_equal(idx.cpu().numpy(), ri)
+    np.testing.assert_array_equal(dist.cpu().numpy(), rd.astype(np.float32))
+
+
+def test_knn_fp32_slices_and_self(cuda):
+    import torch
+    from scarf_amd.knn import pack_refs, search
+    R = _rand_embedding(6000, 50, 7, dup_rows=30)
+    Rt = torch.from_numpy(R).cuda()
+    refs = pack_refs(Rt, "fp32")
+    for S in (1, 3, 16):
+        say("knn self-query slices", S)
+        idx, dist, flags = search(Rt[:900], Rt, refs, 11, pool=32, self_offset=0, n_slices=S)
+        fi, fd, nm, _ = oknn.self_query_knn(R, 11, query_rows=np.arange(900))
+        np.testing.assert_array_equal(idx.cpu().numpy(), fi)
+        np.testing.assert_array_equal(dist.cpu().numpy(), fd.astype(np.float32))
+        got_mis = int(((flags.cpu().numpy() & 2) > 0).sum())
+        # misses = rows whose first neighbour is not themselves (ann.py:34-35); the
+        # flag marks those that had to drop their last column; compare the latter
+        ki, _ = oknn.knn_f64(R[:900], R, 12)
+        drop = sum(1 for i in range(900) if i not in ki[i])
+        assert got_mis == drop
+
+
+def test_knn_bf16_recall(cuda):
+    import torch
+    from scarf_amd.knn import pack_refs, search
+    R = _rand_embedding(20000, 50, 8)
+    Q = R[:2000]
+    Rt, Qt = torch.from_numpy(R).cuda(), torch.from_numpy(Q).cuda()
+    say("knn bf16 pool 64")
+    refs = pack_refs(Rt, "bf16")
+    idx, dist, _ = search(Qt, Rt, refs, 11, pool=64)
+    ri, rd = oknn.knn_f64(Q, R, 11)
+    got = idx.cpu().numpy()
+    recall = np.mean([len(set(got[i]) & set(ri[i])) / 11 for i in range(len(Q))])
+    say("bf16 recall@11", recall)
+    assert recall >= 0.999
+    # returned distances are exact FP64 values of the returned neighbours
+    pd = oknn.pair_dist_f64(Q, R, np.repeat(np.arange(len(Q)), 11), got.ravel())
+    np.testing.assert_array_equal(dist.cpu().numpy().ravel(), pd.astype(np.float32))
+
+
+def test_knn_exact_kernel(cuda):
+    import torch
+    from scarf_amd.knn import exact_knn
+    R = _rand_embedding(3000, 50, 9, dup_rows=20)
+    Rt = torch.from_numpy(R).cuda()
+    say("knn exact fp64 brute force")
+    idx, dist, d64, _ = exact_knn(Rt[:500], Rt, 21)
+    ri, rd = oknn.knn_f64(R[:500], R, 21)
+    np.testing.assert_array_equal(idx.cpu().numpy(), ri)
+    np.testing.assert_array_equal(d64.cpu().numpy(), rd)
+
+
+def test_smooth_reference_fixture_gpu(cuda):
+    import torch
+    from scarf_amd.smooth import smooth_knn
+    ki = np.load(os.path.join(GOLD, "knn_indices.npy")).astype(np.int64)
+    kd = np.load(os.path.join(GOLD, "knn_distances.npy"))
+    kw = np.load(os.path.join(GOLD, "knn_weights.npy"))
+    say("smooth fixture")
+    _, _, w = smooth_knn(torch.from_numpy(kd.astype(np.float32)).cuda(),
+                         torch.from_numpy(ki).cuda(), 1000, 1.0, 1.5)
+    w = w.cpu().numpy()
+    assert np.abs(w - kw).max() < 1e-5
+
+
+def test_smooth_chunked_vs_oracle(cuda):
+    import torch
+    from scarf_amd.smooth import smooth_knn
+    R = _rand_embedding(2500, 20, 10, dup_rows=5)
+    fi, fd, _, _ = oknn.self_query_knn(R, 11)
+    say("smooth chunked 2500 rows chunk 300")
+    _, wr = osmooth.smoothen_dists(fi, fd, 1.0, 1.5, 300)
+    _, _, w = smooth_knn(torch.from_numpy(fd.astype(np.float32)).cuda(),
+                         torch.from_numpy(fi).cuda(), 300, 1.0, 1.5)
+    assert np.abs(w.cpu().numpy() - wr).max() < 1e-5
+
+
+def test_mapping_scores_gpu(cuda):
+    import torch
+    from scarf_amd.mapping import mapping_scores
+    ki = np.load(os.path.join(GOLD, "knn_indices.npy")).astype(np.int64)
+    kd = np.load(os.path.join(GOLD, "knn_distances.npy"))
+    fx = np.load(os.path.join(GOLD, "mapping_scores.npy"))
+    n = ki.shape[0]
+    idx = np.stack([np.arange(n), ki[:, 0], ki[:, 1]], 1)
+    dist = np.stack([np.zeros(n), kd[:, 0], kd[:, 1]], 1).astype(np.float32)
+    say("mapping scores fixture")
+    (_, ms), = mapping_scores(torch.from_numpy(idx).cuda(), torch.from_numpy(dist).cuda(), n,
+                              multiplier=3000.0)
+    assert np.abs(ms - fx).max() < 1e-6  # float32 distances, as scarf stores hnswlib's
+    groups = np.random.default_rng(0).integers(0, 3, n)
+    got = mapping_scores(torch.from_numpy(idx).cuda(), torch.from_numpy(dist).cuda(), n,
+                         groups=groups)
+    want = omap.mapping_scores(idx, dist.astype(np.float64), n, target_groups=groups)
+    for (g1, m1), (g2, m2) in zip(got, want):
+        assert g1 == g2
+        np.testing.assert_allclose(m1, m2, rtol=1e-11, atol=1e-12)
