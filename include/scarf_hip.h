@@ -45,6 +45,7 @@ extern "C" {
 /* Bits of the per-query flags written by sh_knn_rerank / sh_knn_exact. */
 #define SH_FLAG_UNCERTIFIED 1 /* FP32 screen could not prove its top-k exact */
 #define SH_FLAG_SELF_MISSING 2 /* fix_knn_query had to drop the last neighbour */
+#define SH_FLAG_SELF_NOT_FIRST 4 /* column 0 was not the query (a miss, ann.py:34-35) */
 
 typedef void *sh_stream_t;
 

@@ -23,6 +23,7 @@ DTYPES = {"fp32": SH_F32, "f32": SH_F32, "bf16": SH_BF16, "fp8": SH_FP8}
 
 SH_FLAG_UNCERTIFIED = 1
 SH_FLAG_SELF_MISSING = 2
+SH_FLAG_SELF_NOT_FIRST = 4
 
 _p = ctypes.c_void_p
 _i32 = ctypes.c_int32

@@ -455,6 +455,7 @@ __global__ void __launch_bounds__(256) k_rerank(
     } else {
       drop = pos;
     }
+    if (pos != 0) flag |= SH_FLAG_SELF_NOT_FIRST;
   }
   // output column col takes element col (col < drop) or col + 1 (col >= drop); the
   // element may sit in another lane: shuffle
@@ -535,6 +536,7 @@ __global__ void __launch_bounds__(256) k_exact(
     }
     drop = pos < 0 ? kk - 1 : pos;
     if (pos < 0) flag |= SH_FLAG_SELF_MISSING;
+    if (pos != 0) flag |= SH_FLAG_SELF_NOT_FIRST;
   }
   for (int col = 0; col < k; ++col) {
     const int e = (drop >= 0 && col >= drop) ? col + 1 : col;

@@ -34,6 +34,40 @@ except ImportError:  # pragma: no cover
 
 U32 = 2.0 ** -24
 
+# Optional per-launch HIP-event timing of the hot kernels (enabled by measurement
+# harnesses): name -> list of (start event, end event, algorithmic work).
+_timing = None
+
+
+def enable_kernel_timing():
+    global _timing
+    _timing = {}
+    return _timing
+
+
+def disable_kernel_timing():
+    global _timing
+    t, _timing = _timing, None
+    return t
+
+
+class kernel_timer:
+    def __init__(self, name, stream=None, flops=0.0, nbytes=0.0):
+        self.name, self.stream, self.flops, self.nbytes = name, stream, flops, nbytes
+
+    def __enter__(self):
+        if _timing is not None:
+            self.s = torch.cuda.Event(enable_timing=True)
+            self.s.record(self.stream or torch.cuda.current_stream())
+        return self
+
+    def __exit__(self, *exc):
+        if _timing is not None:
+            e = torch.cuda.Event(enable_timing=True)
+            e.record(self.stream or torch.cuda.current_stream())
+            _timing.setdefault(self.name, []).append((self.s, e, self.flops, self.nbytes))
+        return False
+
 
 @dataclasses.dataclass
 class PackedRefs:
@@ -116,8 +150,9 @@ def search(Q: "torch.Tensor", R: "torch.Tensor", refs: PackedRefs, k: int, pool:
     cand = torch.empty((nq, S, pool), dtype=torch.int64, device=dev)
     cnt = torch.empty((nq, S), dtype=torch.int32, device=dev)
     ws, nbytes = workspace("screen").get(_lib.lib().sh_knn_screen_workspace(nq, pool, S))
-    _lib.call("sh_knn_screen", ptr(qp), nq, ptr(refs.data), nr, D, code, None, None, pool, S,
-              ptr(cand), ptr(cnt), ptr(ws), nbytes, st)
+    with kernel_timer("screen", stream, flops=2.0 * nq * nr * D):
+        _lib.call("sh_knn_screen", ptr(qp), nq, ptr(refs.data), nr, D, code, None, None, pool,
+                  S, ptr(cand), ptr(cnt), ptr(ws), nbytes, st)
     if certify is None:
         certify = refs.dtype in ("fp32", "f32")
     err_coef = -1.0
