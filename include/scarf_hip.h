@@ -132,16 +132,26 @@ int sh_knn_screen(const void *qpack, int64_t nq, const void *rpack, int64_t nr,
                   int32_t *cand_cnt, void *workspace, size_t workspace_bytes,
                   sh_stream_t stream);
 size_t sh_knn_screen_workspace(int64_t nq, int32_t pool, int32_t n_slices);
+/* Self queries: query q is reference self_idx[q] (self_idx non-NULL) or
+ * self_offset + q (self_offset >= 0); k + 1 neighbours are ranked and the query's
+ * own column removed as fix_knn_query does.  Neither: a plain k-neighbour query. */
 int sh_knn_rerank(const float *Q, int64_t nq, int64_t ldq, const float *R, int64_t nr,
                   int64_t ldr, int32_t D, const uint64_t *cand_keys,
                   const int32_t *cand_cnt, int32_t pool, int32_t n_slices,
-                  int32_t k, int64_t self_offset, double err_coef, float r_norm_max,
-                  int64_t *idx_out, float *dist_out, int32_t *flags_out,
-                  sh_stream_t stream);
+                  int32_t k, int64_t self_offset, const int64_t *self_idx,
+                  double err_coef, float r_norm_max, int64_t *idx_out, float *dist_out,
+                  int32_t *flags_out, sh_stream_t stream);
 int sh_knn_exact(const float *Q, int64_t nq, int64_t ldq, const int64_t *q_rows,
                  int64_t n_sel, const float *R, int64_t nr, int64_t ldr, int32_t D,
-                 int32_t k, int64_t self_offset, int64_t *idx_out, float *dist_out,
-                 double *dist64_out, int32_t *flags_out, sh_stream_t stream);
+                 int32_t k, int64_t self_offset, const int64_t *self_idx,
+                 int64_t *idx_out, float *dist_out, double *dist64_out,
+                 int32_t *flags_out, sh_stream_t stream);
+/* fix_knn_query on an existing (k+1)-column table [n x k1]: removes each row's own
+ * index (ref_idx[row]) or, if absent, its last column.  Outputs [n x (k1-1)].
+ * Replaces: ann.py:31-52 (fix_knn_query). */
+int sh_fix_knn_query(const int64_t *idx_in, const float *dist_in, const int64_t *ref_idx,
+                     int64_t n, int32_t k1, int64_t *idx_out, float *dist_out,
+                     int32_t *flags_out, sh_stream_t stream);
 
 /* ---- fuzzy edge weights ---------------------------------------------------------
  * umap smooth_knn_dist + compute_membership_strengths per `chunk_size` row chunk,

@@ -131,7 +131,7 @@ __global__ void __launch_bounds__(256) k_gram(const float *__restrict__ Z, int64
 
 // FP64 variant: v_mfma_f64_16x16x4f64, each wave a 64 x 64 block = 4 x 4 MFMA tiles
 // of 16 x 16.  A/B operand: lane l supplies row k = l/16 of the 4-row step and
-// column l%16; D: lane l holds rows 4*(l/16) + i (i < 4) of column l%16.
+// column l%16; D: lane l holds rows 4*i + l/16 (i < 4, interleaved) of column l%16.
 typedef double doublex4 __attribute__((ext_vector_type(4)));
 __global__ void __launch_bounds__(256) k_gram64(const double *__restrict__ Z, int64_t n,
                                                 int64_t ld, int nb, int T, int S,
@@ -214,7 +214,7 @@ __global__ void __launch_bounds__(256) k_gram64(const double *__restrict__ Z, in
     for (int c = 0; c < 4; ++c)
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        const int m = wm * 64 + a * 16 + 4 * (lane >> 4) + i;
+        const int m = wm * 64 + a * 16 + 4 * i + (lane >> 4);
         const int nn = wn * 64 + c * 16 + (lane & 15);
         dst[m * kGT + nn] = acc[a][c][i];
       }

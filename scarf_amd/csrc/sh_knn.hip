@@ -307,6 +307,7 @@ __global__ void __launch_bounds__(256) k_rerank(
     const float *__restrict__ Q, int64_t nq, int64_t ldq, const float *__restrict__ R,
     int64_t nr, int64_t ldr, int32_t D, const uint64_t *__restrict__ cand,
     const int32_t *__restrict__ cand_cnt, int pool, int S, int k, int64_t self_offset,
+    const int64_t *__restrict__ self_idx,
     double err_coef, float r_norm_max, int64_t *__restrict__ idx_out,
     float *__restrict__ dist_out, int32_t *__restrict__ flags) {
   __shared__ float qv[4][256];
@@ -382,7 +383,7 @@ __global__ void __launch_bounds__(256) k_rerank(
   }
   wave_bitonic_sort<PE>(c);
 
-  const bool self = self_offset >= 0;
+  const bool self = self_idx != nullptr || self_offset >= 0;
   const int kk = self ? k + 1 : k;
   // broadcast the first kk (<= 64*PE) sorted elements through LDS-free shuffles
   // element e is at lane e/PE, slot e%PE
@@ -440,7 +441,7 @@ __global__ void __launch_bounds__(256) k_rerank(
   // self repair (ann.py:31-52) on the kk = k+1 list
   int drop = -1;
   if (self) {
-    const int64_t me = self_offset + q;
+    const int64_t me = self_idx ? self_idx[q] : self_offset + q;
     int pos = -1;
     for (int e = 0; e < kk; ++e) {
       const uint32_t ie = (uint32_t)__shfl((int)ii[e / 64], e % 64, kWave);
@@ -485,6 +486,7 @@ template <int KK>
 __global__ void __launch_bounds__(256) k_exact(
     const float *__restrict__ Q, int64_t ldq, const int64_t *__restrict__ q_rows, int64_t n_sel,
     const float *__restrict__ R, int64_t nr, int64_t ldr, int32_t D, int k, int64_t self_offset,
+    const int64_t *__restrict__ self_idx,
     int64_t *__restrict__ idx_out, float *__restrict__ dist_out, double *__restrict__ dist64_out,
     int32_t *__restrict__ flags) {
   __shared__ float qv[4][256];
@@ -516,12 +518,12 @@ __global__ void __launch_bounds__(256) k_exact(
     }
   }
   wave_bitonic_sort<KK>(best);
-  const bool self = self_offset >= 0;
+  const bool self = self_idx != nullptr || self_offset >= 0;
   const int kk = self ? k + 1 : k;
   int drop = -1;
   int32_t flag = 0;
   if (self) {
-    const int64_t me = self_offset + q;
+    const int64_t me = self_idx ? self_idx[q] : self_offset + q;
     int pos = -1;
     for (int e = 0; e < kk; ++e) {
       uint32_t mine = 0;
@@ -559,9 +561,49 @@ __global__ void __launch_bounds__(256) k_exact(
   if (flags && lane == 0) flags[q] = flag;
 }
 
+
+// fix_knn_query (ann.py:31-52) on a (k+1)-column table, one thread per row: drop the
+// query's own column (column 0 in the common case), else the last column.
+__global__ void k_fix_knn(const int64_t *__restrict__ idx_in, const float *__restrict__ dist_in,
+                          const int64_t *__restrict__ ref_idx, int64_t n, int k1,
+                          int64_t *__restrict__ idx_out, float *__restrict__ dist_out,
+                          int32_t *__restrict__ flags) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t me = ref_idx[i];
+  int pos = -1;
+  for (int e = 0; e < k1; ++e)
+    if (idx_in[i * k1 + e] == me) {
+      pos = e;
+      break;
+    }
+  int32_t flag = 0;
+  if (pos != 0) flag |= SH_FLAG_SELF_NOT_FIRST;
+  if (pos < 0) flag |= SH_FLAG_SELF_MISSING;
+  const int drop = pos < 0 ? k1 - 1 : pos;
+  for (int c = 0; c < k1 - 1; ++c) {
+    const int e = c < drop ? c : c + 1;
+    idx_out[i * (k1 - 1) + c] = idx_in[i * k1 + e];
+    dist_out[i * (k1 - 1) + c] = dist_in[i * k1 + e];
+  }
+  if (flags) flags[i] = flag;
+}
+
 }  // namespace sh
 
 using namespace sh;
+
+extern "C" int sh_fix_knn_query(const int64_t *idx_in, const float *dist_in,
+                                const int64_t *ref_idx, int64_t n, int32_t k1, int64_t *idx_out,
+                                float *dist_out, int32_t *flags_out, sh_stream_t stream) {
+  SH_ARG(n >= 0 && k1 >= 2, "sh_fix_knn_query: need n >= 0 and at least 2 columns");
+  SH_ARG(idx_in && dist_in && ref_idx && idx_out && dist_out, "sh_fix_knn_query: NULL pointer");
+  if (n == 0) return SH_OK;
+  hipLaunchKernelGGL(k_fix_knn, dim3(cdiv(n, 256)), dim3(256), 0, (hipStream_t)stream, idx_in,
+                     dist_in, ref_idx, n, k1, idx_out, dist_out, flags_out);
+  SH_LAUNCH_CHECK("k_fix_knn");
+  return SH_OK;
+}
 
 extern "C" int32_t sh_knn_pack_k(int dtype, int32_t D) { return pack_k(dtype, D); }
 
@@ -678,12 +720,13 @@ extern "C" int sh_knn_screen(const void *qpack, int64_t nq, const void *rpack, i
 extern "C" int sh_knn_rerank(const float *Q, int64_t nq, int64_t ldq, const float *R, int64_t nr,
                              int64_t ldr, int32_t D, const uint64_t *cand_keys,
                              const int32_t *cand_cnt, int32_t pool, int32_t n_slices, int32_t k,
-                             int64_t self_offset, double err_coef, float r_norm_max,
+                             int64_t self_offset, const int64_t *self_idx, double err_coef,
+                             float r_norm_max,
                              int64_t *idx_out, float *dist_out, int32_t *flags_out,
                              sh_stream_t stream) {
   SH_ARG(nq >= 0 && nr > 0 && D > 0 && D <= 256, "sh_knn_rerank: bad sizes");
   SH_ARG(pool >= 1 && pool <= 128 && n_slices >= 1, "sh_knn_rerank: bad pool/slices");
-  const int kk = k + (self_offset >= 0 ? 1 : 0);
+  const int kk = k + ((self_offset >= 0 || self_idx) ? 1 : 0);
   SH_ARG(k >= 1 && kk <= pool && kk <= 127, "sh_knn_rerank: k=%d (+self) must be <= pool=%d", k,
          pool);
   SH_ARG(Q && R && cand_keys && cand_cnt && idx_out && dist_out, "sh_knn_rerank: NULL pointer");
@@ -691,11 +734,11 @@ extern "C" int sh_knn_rerank(const float *Q, int64_t nq, int64_t ldq, const floa
   dim3 grid(cdiv(nq, 4)), block(256);
   if (pool <= 64)
     hipLaunchKernelGGL(k_rerank<1>, grid, block, 0, (hipStream_t)stream, Q, nq, ldq, R, nr, ldr,
-                       D, cand_keys, cand_cnt, pool, n_slices, k, self_offset, err_coef,
+                       D, cand_keys, cand_cnt, pool, n_slices, k, self_offset, self_idx, err_coef,
                        r_norm_max, idx_out, dist_out, flags_out);
   else
     hipLaunchKernelGGL(k_rerank<2>, grid, block, 0, (hipStream_t)stream, Q, nq, ldq, R, nr, ldr,
-                       D, cand_keys, cand_cnt, pool, n_slices, k, self_offset, err_coef,
+                       D, cand_keys, cand_cnt, pool, n_slices, k, self_offset, self_idx, err_coef,
                        r_norm_max, idx_out, dist_out, flags_out);
   SH_LAUNCH_CHECK("k_rerank");
   return SH_OK;
@@ -703,12 +746,13 @@ extern "C" int sh_knn_rerank(const float *Q, int64_t nq, int64_t ldq, const floa
 
 extern "C" int sh_knn_exact(const float *Q, int64_t nq, int64_t ldq, const int64_t *q_rows,
                             int64_t n_sel, const float *R, int64_t nr, int64_t ldr, int32_t D,
-                            int32_t k, int64_t self_offset, int64_t *idx_out, float *dist_out,
+                            int32_t k, int64_t self_offset, const int64_t *self_idx,
+                            int64_t *idx_out, float *dist_out,
                             double *dist64_out, int32_t *flags_out, sh_stream_t stream) {
   (void)nq;
   SH_ARG(n_sel >= 0 && nr > 0 && D > 0 && D <= 256, "sh_knn_exact: bad sizes");
   SH_ARG(nr < 0xffffffffll, "sh_knn_exact: more than 2^32-1 references");
-  const int kk = k + (self_offset >= 0 ? 1 : 0);
+  const int kk = k + ((self_offset >= 0 || self_idx) ? 1 : 0);
   SH_ARG(k >= 1 && kk <= 64, "sh_knn_exact: k=%d (+self) must be <= 64", k);
   SH_ARG(Q && R && idx_out && dist_out, "sh_knn_exact: NULL pointer");
   if (n_sel == 0) return SH_OK;
@@ -716,13 +760,13 @@ extern "C" int sh_knn_exact(const float *Q, int64_t nq, int64_t ldq, const int64
   hipStream_t st = (hipStream_t)stream;
   if (kk <= 16)
     hipLaunchKernelGGL(k_exact<16>, grid, block, 0, st, Q, ldq, q_rows, n_sel, R, nr, ldr, D, k,
-                       self_offset, idx_out, dist_out, dist64_out, flags_out);
+                       self_offset, self_idx, idx_out, dist_out, dist64_out, flags_out);
   else if (kk <= 32)
     hipLaunchKernelGGL(k_exact<32>, grid, block, 0, st, Q, ldq, q_rows, n_sel, R, nr, ldr, D, k,
-                       self_offset, idx_out, dist_out, dist64_out, flags_out);
+                       self_offset, self_idx, idx_out, dist_out, dist64_out, flags_out);
   else
     hipLaunchKernelGGL(k_exact<64>, grid, block, 0, st, Q, ldq, q_rows, n_sel, R, nr, ldr, D, k,
-                       self_offset, idx_out, dist_out, dist64_out, flags_out);
+                       self_offset, self_idx, idx_out, dist_out, dist64_out, flags_out);
   SH_LAUNCH_CHECK("k_exact");
   return SH_OK;
 }

@@ -1,0 +1,206 @@
+"""Drop-ins for ``DataStore.make_graph``, ``run_mapping`` and ``get_mapping_score``.
+
+The reference keeps every result in a zarr hierarchy whose group names carry the
+parameters (``graph_datastore.py:558-575``, ``738-745``).  These functions take the
+counts as CSR (host ``scarf_amd.synth.CSR`` or ``DeviceCSR``) and write the same
+datasets under the same paths into ``store`` (any mutable mapping; a dict by default):
+
+  {assay}/normed__{cell_key}__{feat_key}/mu, /sigma
+  .../reduction__pca__{dims}__{pca_cell_key}/reduction                (loadings)
+  .../ann__{metric}__{efc}__{ef}__{M}__{seed}/knn__{k}/indices, /distances
+  .../knn__{k}/graph__{lc}__{bw}/edges, /weights
+  {assay}/projections/{target_name}/indices, /distances
+"""
+from __future__ import annotations
+
+from typing import Dict, Optional, Sequence
+
+import numpy as np
+
+from .ann import AnnStream, NormedCSR
+from .device import DeviceCSR, col_map_from_positions, require_gpu
+from .knn_utils import self_query_knn, smoothen_dists
+from .mapping import align_col_map, mapping_scores
+from .normalize import NormParams, feature_stats
+from .pca import Projector
+
+try:
+    import torch
+except ImportError:  # pragma: no cover
+    torch = None
+
+DEFAULTS = {"dims": 11, "k": 11, "ann_metric": "l2", "rand_state": 4466,
+            "n_centroids": 1000, "local_connectivity": 1.0, "bandwidth": 1.5}
+
+
+def _device_csr(counts) -> DeviceCSR:
+    if isinstance(counts, DeviceCSR):
+        return counts
+    return DeviceCSR.from_host(counts)
+
+
+def graph_locations(assay, cell_key, feat_key, dims, pca_cell_key, metric, efc, ef, m, seed,
+                    k, lc, bw):
+    normed = f"{assay}/normed__{cell_key}__{feat_key}"
+    reduction = f"{normed}/reduction__pca__{dims}__{pca_cell_key}"
+    ann = f"{reduction}/ann__{metric}__{efc}__{ef}__{m}__{seed}"
+    knn = f"{ann}/knn__{k}"
+    graph = f"{knn}/graph__{lc}__{bw}"
+    return normed, reduction, ann, knn, graph
+
+
+def make_graph(counts, feat_idx=None, *, assay: str = "RNA", cell_key: str = "I",
+               feat_key: str = "hvgs", pca_cell_key: Optional[str] = None,
+               use_for_pca: Optional[np.ndarray] = None, dims: Optional[int] = None,
+               k: Optional[int] = None, ann_metric: str = "l2", ann_efc: Optional[int] = None,
+               ann_ef: Optional[int] = None, ann_m: Optional[int] = None,
+               rand_state: int = 4466, batch_size: int = 1000, size_factor: float = 1000.0,
+               log_transform: bool = True, renormalize_subset: bool = True,
+               n_counts=None, local_connectivity: float = 1.0, bandwidth: float = 1.5,
+               feat_scaling: bool = True, custom_loadings: Optional[np.ndarray] = None,
+               clip: Optional[float] = None, return_ann_object: bool = False,
+               screen: str = "bf16", pool: int = 64, pca_mode: str = "ipca",
+               store: Optional[Dict] = None):
+    """``GraphDataStore.make_graph`` (``graph_datastore.py:513-1020``) on CSR counts.
+
+    Parameter defaults follow ``_set_graph_params`` (``graph_datastore.py:139-149``,
+    ``272``, ``298-303``).  Returns the AnnStream if ``return_ann_object`` else the store.
+    """
+    require_gpu()
+    store = {} if store is None else store
+    if ann_metric != "l2":
+        raise ValueError("only ann_metric='l2' is supported")
+    dims = DEFAULTS["dims"] if dims is None else int(dims)
+    method = "pca"
+    if custom_loadings is not None:
+        method = "custom"
+        dims = custom_loadings.shape[1]
+    k = DEFAULTS["k"] if k is None else int(k)
+    ann_m = min(max(48, int(dims * 1.5)), 64) if ann_m is None else int(ann_m)
+    ann_ef = min(100, max(k * 3, 50)) if ann_ef is None else int(ann_ef)
+    ann_efc = min(100, max(k * 3, 50)) if ann_efc is None else int(ann_efc)
+    pca_cell_key = cell_key if pca_cell_key is None else pca_cell_key
+    csr = _device_csr(counts)
+    norm = NormParams(size_factor, log_transform, renormalize_subset, clip)
+    nc = None
+    if n_counts is not None:
+        nc = torch.as_tensor(np.asarray(n_counts, dtype=np.float64)).cuda()
+    data = NormedCSR(csr, norm, batch_size=batch_size, feat_idx=feat_idx, n_counts=nc)
+    if custom_loadings is not None and data.shape[1] != custom_loadings.shape[0]:
+        raise ValueError(f"Provided custom loadings has {custom_loadings.shape[0]} features "
+                         f"while the data has {data.shape[1]} features.")
+    normed_loc, reduction_loc, ann_loc, knn_loc, graph_loc = graph_locations(
+        assay, cell_key, feat_key, dims, pca_cell_key, ann_metric, ann_efc, ann_ef, ann_m,
+        rand_state, k, local_connectivity, bandwidth)
+    st = data.stats()
+    mu = store.get(normed_loc + "/mu")
+    sigma = store.get(normed_loc + "/sigma")
+    if mu is None:
+        mu = st.mu.cpu().numpy()
+        store[normed_loc + "/mu"] = mu
+    if sigma is None:
+        sigma = st.sigma.cpu().numpy()
+        store[normed_loc + "/sigma"] = sigma
+    loadings = store.get(reduction_loc + "/reduction")
+    if loadings is not None and loadings.shape[0] != data.shape[1]:
+        loadings = None
+    if custom_loadings is not None:
+        loadings = np.asarray(custom_loadings, dtype=np.float64)
+    ann_idx = store.get(ann_loc + "/ann_idx")
+    ann = AnnStream(data=data, k=k, n_cluster=DEFAULTS["n_centroids"],
+                    reduction_method=method, dims=dims, loadings=loadings,
+                    use_for_pca=use_for_pca, mu=mu, sigma=sigma, ann_metric=ann_metric,
+                    ann_efc=ann_efc, ann_ef=ann_ef, ann_m=ann_m, nthreads=1,
+                    ann_parallel=False, rand_state=rand_state, do_kmeans_fit=False,
+                    disable_scaling=not feat_scaling, ann_idx=ann_idx, lsi_skip_first=True,
+                    lsi_params={}, harmonize=False, screen=screen, pool=pool,
+                    pca_mode=pca_mode)
+    store[reduction_loc + "/reduction"] = ann.loadings
+    store[ann_loc + "/ann_idx"] = ann.annIdx
+    if knn_loc + "/indices" not in store or graph_loc + "/weights" not in store:
+        recall = None
+        if knn_loc + "/indices" not in store:
+            sub = {}
+            recall = self_query_knn(ann, sub, batch_size)
+            store[knn_loc + "/indices"] = sub["indices"]
+            store[knn_loc + "/distances"] = sub["distances"]
+            store[knn_loc + "/recall"] = recall
+        g = {}
+        smoothen_dists(g, store[knn_loc + "/indices"], store[knn_loc + "/distances"],
+                       local_connectivity, bandwidth, batch_size)
+        store[graph_loc + "/edges"] = g["edges"]
+        store[graph_loc + "/weights"] = g["weights"]
+    store["latest_graph"] = graph_loc
+    store["latest_knn"] = knn_loc
+    if return_ann_object:
+        return ann
+    return store
+
+
+def run_mapping(ann: AnnStream, target_counts, target_ids: Sequence, source_feat_ids: Sequence,
+                target_name: str, save_k: int = 3, ref_mu: bool = True, ref_sigma: bool = True,
+                target_n_counts=None, store: Optional[Dict] = None, assay: str = "RNA"):
+    """``MappingDatastore.run_mapping`` (``mapping_datastore.py:31-209``), CORAL off.
+
+    The target uses the reference's size factor and subset parameters
+    (``mapping_datastore.py:106-111``, ``mapping_utils.py:188-194``); features are
+    aligned to the reference order with missing ones set to 1.0
+    (``mapping_utils.py:197-213``); neighbours are the k = save_k nearest reference
+    cells with no self repair (``mapping_datastore.py:198-208``).
+    """
+    require_gpu()
+    store = {} if store is None else store
+    if save_k > ann.k:
+        save_k = ann.k
+    tcsr = _device_csr(target_counts)
+    positions, missing = align_col_map(list(source_feat_ids), list(target_ids))
+    if len(positions) != tcsr.n_cols:
+        raise ValueError("target_ids must name every column of the target matrix")
+    cm = col_map_from_positions(tcsr.n_cols, positions)
+    d = len(source_feat_ids)
+    norm = ann.data.norm
+    libsize = None
+    nct = None
+    if not norm.renormalize_subset:
+        if target_n_counts is None:
+            raise ValueError("renormalize_subset=False needs the target's per-cell totals")
+        nct = torch.as_tensor(np.asarray(target_n_counts, dtype=np.float64)).cuda()
+        libsize = nct
+    mu, sigma = ann.mu, ann.sigma
+    if not (ref_mu and ref_sigma):
+        tst = feature_stats(tcsr, 1000, norm, col_map=cm, d=d, n_counts=nct)
+        tm = tst.mu.cpu().numpy()
+        ts = tst.sigma.cpu().numpy()
+        # aligned columns that are missing in the target are constant 1.0 there
+        tm[missing] = 1.0
+        ts[missing] = 1.0
+        if not ref_mu:
+            mu = tm
+        if not ref_sigma:
+            sigma = ts
+    proj = Projector.build(ann.loadings, mu, sigma, clip=norm.clip,
+                           disable_scaling=ann._disable_scaling, fill_missing=missing)
+    E = proj.project(tcsr, norm, libsize=libsize, col_map=cm)
+    idx, dist, _ = ann.annIdx.knn_query_tensor(E, save_k)
+    loc = f"{assay}/projections/{target_name}"
+    store[loc + "/indices"] = idx.cpu().numpy().astype(np.uint64)
+    store[loc + "/distances"] = dist.cpu().numpy().astype(np.float64)
+    store[loc + "/_ref_n_cells"] = ann.nCells
+    return store
+
+
+def get_mapping_score(store: Dict, target_name: str, target_groups=None,
+                      log_transform: bool = True, multiplier: float = 1000,
+                      weighted: bool = True, fixed_weight: float = 0.1, assay: str = "RNA",
+                      ref_n_cells: Optional[int] = None):
+    """Generator of (group, scores) like ``get_mapping_score`` (``mapping_datastore.py:211-285``)."""
+    loc = f"{assay}/projections/{target_name}"
+    if loc + "/indices" not in store:
+        raise KeyError(f"ERROR: Projections have not been computed for {target_name}")
+    idx = torch.from_numpy(np.asarray(store[loc + "/indices"]).astype(np.int64)).cuda()
+    dist = torch.from_numpy(np.asarray(store[loc + "/distances"], dtype=np.float32)).cuda()
+    n_ref = int(ref_n_cells if ref_n_cells is not None else store[loc + "/_ref_n_cells"])
+    for g, ms in mapping_scores(idx, dist, n_ref, groups=target_groups,
+                                log_transform=log_transform, multiplier=multiplier,
+                                weighted=weighted, fixed_weight=fixed_weight):
+        yield g, ms

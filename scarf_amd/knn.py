@@ -121,7 +121,8 @@ def _fix_slices(nr: int, s: int) -> int:
 
 def search(Q: "torch.Tensor", R: "torch.Tensor", refs: PackedRefs, k: int, pool: int = 64,
            self_offset: Optional[int] = None, n_slices: Optional[int] = None,
-           certify: Optional[bool] = None, stream=None):
+           certify: Optional[bool] = None, self_idx: Optional["torch.Tensor"] = None,
+           stream=None):
     """Top-k of each query row of ``Q`` against ``R`` (both FP32 [*, D] on device).
 
     Returns (idx int64 [nq, k], dist f32 [nq, k], flags int32 [nq]).
@@ -132,7 +133,12 @@ def search(Q: "torch.Tensor", R: "torch.Tensor", refs: PackedRefs, k: int, pool:
     dev = Q.device
     if D != refs.D:
         raise ValueError(f"query dim {D} != index dim {refs.D}")
-    kk = k + (1 if self_offset is not None else 0)
+    has_self = self_offset is not None or self_idx is not None
+    if self_idx is not None:
+        self_idx = self_idx.to(device=Q.device, dtype=torch.int64).contiguous()
+        if self_idx.numel() != nq:
+            raise ValueError("self_idx needs one reference index per query row")
+    kk = k + (1 if has_self else 0)
     if kk > nr:
         raise ValueError(f"k={k} needs at least {kk} reference rows, index has {nr}")
     pool = max(int(pool), kk)
@@ -161,22 +167,37 @@ def search(Q: "torch.Tensor", R: "torch.Tensor", refs: PackedRefs, k: int, pool:
         err_coef = 2.0 * (refs.K + 2) * U32
     so = -1 if self_offset is None else int(self_offset)
     _lib.call("sh_knn_rerank", ptr(Q), nq, Q.stride(0), ptr(R), nr, R.stride(0), D, ptr(cand),
-              ptr(cnt), pool, S, k, so, float(err_coef), float(refs.norm_max), ptr(idx),
-              ptr(dist), ptr(flags), st)
+              ptr(cnt), pool, S, k, so, ptr(self_idx), float(err_coef), float(refs.norm_max),
+              ptr(idx), ptr(dist), ptr(flags), st)
     if certify:
         bad = torch.nonzero(flags[:nq] & _lib.SH_FLAG_UNCERTIFIED).flatten()
         if bad.numel():
-            exact_rows(Q, R, k, bad, idx, dist, flags, self_offset, stream)
+            exact_rows(Q, R, k, bad, idx, dist, flags, self_offset, stream, self_idx=self_idx)
     return idx, dist, flags[:nq]
 
 
-def exact_rows(Q, R, k, rows, idx, dist, flags, self_offset=None, stream=None, dist64=None):
+def exact_rows(Q, R, k, rows, idx, dist, flags, self_offset=None, stream=None, dist64=None,
+               self_idx=None):
     """FP64 brute force for the query rows listed in ``rows`` (device int64)."""
     nq, D = Q.shape
     so = -1 if self_offset is None else int(self_offset)
     _lib.call("sh_knn_exact", ptr(Q), nq, Q.stride(0), ptr(rows), int(rows.numel()), ptr(R),
-              R.shape[0], R.stride(0), D, k, so, ptr(idx), ptr(dist), ptr(dist64), ptr(flags),
-              stream_handle(stream))
+              R.shape[0], R.stride(0), D, k, so, ptr(self_idx), ptr(idx), ptr(dist), ptr(dist64),
+              ptr(flags), stream_handle(stream))
+
+
+def fix_knn_query_device(idx, dist, ref_idx, stream=None):
+    """fix_knn_query (ann.py:31-52) on device tables: (idx [n, k], dist [n, k], n_mis)."""
+    require_gpu()
+    n, k1 = idx.shape
+    dev = idx.device
+    oi = torch.empty((n, k1 - 1), dtype=torch.int64, device=dev)
+    od = torch.empty((n, k1 - 1), dtype=torch.float32, device=dev)
+    fl = torch.zeros(max(n, 1), dtype=torch.int32, device=dev)
+    _lib.call("sh_fix_knn_query", ptr(idx.contiguous()), ptr(dist.contiguous()),
+              ptr(ref_idx.contiguous()), n, k1, ptr(oi), ptr(od), ptr(fl), stream_handle(stream))
+    n_mis = int(((fl[:n] & _lib.SH_FLAG_SELF_NOT_FIRST) > 0).sum().item())
+    return oi, od, n_mis
 
 
 def exact_knn(Q, R, k, self_offset=None, stream=None):
@@ -190,7 +211,7 @@ def exact_knn(Q, R, k, self_offset=None, stream=None):
     flags = torch.zeros(max(nq, 1), dtype=torch.int32, device=dev)
     so = -1 if self_offset is None else int(self_offset)
     _lib.call("sh_knn_exact", ptr(Q), nq, Q.stride(0), None, nq, ptr(R), R.shape[0],
-              R.stride(0), Q.shape[1], k, so, ptr(idx), ptr(dist), ptr(d64), ptr(flags),
+              R.stride(0), Q.shape[1], k, so, None, ptr(idx), ptr(dist), ptr(d64), ptr(flags),
               stream_handle(stream))
     return idx, dist, d64, flags[:nq]
 
@@ -293,9 +314,11 @@ class GpuIndex:
         return self._packed
 
     # -- query --------------------------------------------------------------------
-    def knn_query_tensor(self, Q: "torch.Tensor", k: int = 1, self_offset=None):
+    def knn_query_tensor(self, Q: "torch.Tensor", k: int = 1, self_offset=None,
+                         self_idx=None):
         """Device in, device out: (int64 idx, f32 squared distances, flags)."""
-        return search(Q, self.embedding(), self.packed(), k, self.pool, self_offset)
+        return search(Q, self.embedding(), self.packed(), k, self.pool, self_offset,
+                      self_idx=self_idx)
 
     def knn_query(self, data, k: int = 1, num_threads: int = -1, filter=None):
         require_gpu()

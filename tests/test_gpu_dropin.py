@@ -1,0 +1,213 @@
+"""GPU: the drop-in layer (make_graph / AnnStream / knn_utils / run_mapping / GpuIndex)
+chained stage by stage against the CPU oracle on the same inputs."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import ipca as oipca
+from oracle import knn as oknn
+from oracle import mapping as omap
+from oracle import norm as onorm
+from oracle import smooth as osmooth
+from scarf_amd.synth import CSR, generate_csr, make_gene_model
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def ref_data():
+    model = make_gene_model(300, 4, 0.12, 31)
+    csr = generate_csr(2400, model, 31)
+    return model, csr
+
+
+def _sk_loadings(x, feat_rows_blocks, dims, mu, sigma):
+    from sklearn.decomposition import IncrementalPCA
+    L, _ = oipca.fit_pca_scarf(feat_rows_blocks, dims, mu, sigma,
+                               fitter=IncrementalPCA(n_components=dims + 1))
+    return L
+
+
+def test_make_graph_chain(cuda, ref_data):
+    from scarf_amd.datastore import make_graph
+    _, csr = ref_data
+    feat = np.sort(np.random.default_rng(1).choice(300, 180, replace=False))
+    say("make_graph chain fp32 screen")
+    store = {}
+    ann = make_graph(csr, feat, dims=20, k=11, batch_size=500, screen="fp32", store=store,
+                     return_ann_object=True)
+    dense = csr.to_dense()
+    x = onorm.normed(dense, sf=1000.0, feat_idx=feat)
+    mu, sigma = onorm.mu_sigma(x)
+    np.testing.assert_allclose(ann.mu, mu, rtol=1e-12, atol=1e-15)
+    np.testing.assert_allclose(ann.sigma, sigma, rtol=1e-12)
+    blocks = [x[s:s + 500] for s in range(0, x.shape[0], 500)]
+    L = _sk_loadings(x, blocks, 20, mu, sigma)
+    Er = ((x - mu) / sigma) @ L
+    E = ann.annIdx.embedding().cpu().numpy().astype(np.float64)
+    for j in range(20):
+        s = np.sign(E[:, j] @ Er[:, j])
+        assert np.linalg.norm(s * E[:, j] - Er[:, j]) / np.linalg.norm(Er[:, j]) < 1e-4
+    np.testing.assert_allclose(ann._pca.explained_variance_ratio_[:20],
+                               oipca.fit_pca_scarf(blocks, 20, mu, sigma)[1].explained_variance_ratio_[:20],
+                               rtol=1e-6)
+    # kNN: oracle on the GPU's own embedding -> bit-exact
+    fi, fd, nm, rec = oknn.self_query_knn(E.astype(np.float32), 11)
+    knn_loc = store["latest_knn"]
+    np.testing.assert_array_equal(store[knn_loc + "/indices"], fi.astype(np.uint64))
+    np.testing.assert_array_equal(store[knn_loc + "/distances"],
+                                  fd.astype(np.float32).astype(np.float64))
+    assert store[knn_loc + "/recall"] == pytest.approx(rec)
+    # weights: oracle smoothing on the same kNN table
+    e_ref, w_ref = osmooth.smoothen_dists(fi, fd.astype(np.float32), 1.0, 1.5, 500)
+    g = store["latest_graph"]
+    np.testing.assert_array_equal(store[g + "/edges"], e_ref)
+    assert np.abs(store[g + "/weights"] - w_ref).max() < 1e-5
+
+
+def test_make_graph_bf16_recall(cuda, ref_data):
+    from scarf_amd.datastore import make_graph
+    _, csr = ref_data
+    say("make_graph bf16 screen")
+    ann = make_graph(csr, None, dims=30, k=21, batch_size=800, screen="bf16",
+                     return_ann_object=True)
+    E = ann.annIdx.embedding().cpu().numpy()
+    fi, _, _, _ = oknn.self_query_knn(E, 21)
+    st = {}
+    from scarf_amd.knn_utils import self_query_knn
+    self_query_knn(ann, st, 800)
+    got = st["indices"].astype(np.int64)
+    rec = np.mean([len(set(got[i]) & set(fi[i])) / 21 for i in range(len(fi))])
+    say("bf16 self recall@21", rec)
+    assert rec >= 0.999
+
+
+def test_run_mapping_and_scores(cuda, ref_data):
+    from scarf_amd.datastore import get_mapping_score, make_graph, run_mapping
+    model, csr = ref_data
+    feat = np.arange(0, 300, 2)  # 150 source features
+    src_ids = [f"g{i}" for i in feat]
+    ann = make_graph(csr, feat, dims=15, k=11, batch_size=600, screen="fp32",
+                     return_ann_object=True)
+    # target: new cells from the same model, columns shuffled, 10 source genes absent,
+    # 20 extra genes the reference does not use
+    tgt = generate_csr(700, model, 77, batch_shift_sigma=0.2)
+    tdense = tgt.to_dense()
+    rng = np.random.default_rng(3)
+    keep = np.setdiff1d(np.arange(300), feat[rng.choice(150, 10, replace=False)])
+    extra = np.setdiff1d(np.arange(300), feat)[:20]
+    cols = np.concatenate([np.intersect1d(keep, feat), extra])
+    rng.shuffle(cols)
+    tcounts = tdense[:, cols].astype(np.uint32)
+    tids = [f"g{i}" for i in cols]
+    say("run_mapping 700 target cells")
+    store = run_mapping(ann, CSR.from_dense(tcounts), tids, src_ids, "t1", save_k=5)
+    # oracle: align, z with reference mu/sigma, project with the GPU loadings
+    t_idx = omap.order_features(src_ids, tids)
+    xa = omap.align_normed(tcounts, t_idx, sf=1000.0)
+    Et = ((xa - ann.mu) / ann.sigma) @ ann.loadings
+    Ef = ann.annIdx.embedding().cpu().numpy()
+    # GPU's target embedding vs oracle (FP32 accumulation)
+    from scarf_amd.pca import Projector
+    i_got = store["RNA/projections/t1/indices"].astype(np.int64)
+    ri, rd = oknn.knn_f64(Et.astype(np.float32), Ef, 5)
+    agree = np.mean([len(set(i_got[i]) & set(ri[i])) / 5 for i in range(len(ri))])
+    say("mapping neighbours agree with oracle-projected targets:", agree)
+    assert agree > 0.99
+    # scores: oracle formula on the stored table
+    (_, ms), = list(get_mapping_score(store, "t1"))
+    (_, msr), = omap.mapping_scores(i_got, store["RNA/projections/t1/distances"], ann.nCells)
+    np.testing.assert_allclose(ms, msr, rtol=1e-10, atol=1e-12)
+
+
+def test_target_projection_parity(cuda, ref_data):
+    """Target embedding itself (alignment + missing = 1.0 + subset library size)."""
+    import torch
+    from scarf_amd.device import DeviceCSR, col_map_from_positions
+    from scarf_amd.mapping import align_col_map
+    from scarf_amd.normalize import NormParams
+    from scarf_amd.pca import Projector
+    model, _ = ref_data
+    tgt = generate_csr(300, model, 78).to_dense()
+    src = [f"g{i}" for i in range(0, 300, 3)]
+    cols = np.arange(1, 300, 2)
+    tids = [f"g{i}" for i in cols]
+    tcounts = tgt[:, cols].astype(np.uint32)
+    rng = np.random.default_rng(5)
+    L = rng.standard_normal((len(src), 12))
+    mu = rng.uniform(0, 1, len(src))
+    sigma = rng.uniform(0.5, 2, len(src))
+    pos, missing = align_col_map(src, tids)
+    proj = Projector.build(L, mu, sigma, fill_missing=missing)
+    dcsr = DeviceCSR.from_host(CSR.from_dense(tcounts))
+    E = proj.project(dcsr, NormParams(), libsize=None,
+                     col_map=col_map_from_positions(dcsr.n_cols, pos)).cpu().numpy()
+    xa = omap.align_normed(tcounts, omap.order_features(src, tids), sf=1000.0)
+    Er = ((xa - mu) / sigma) @ L
+    assert np.abs(E - Er).max() / np.abs(Er).max() < 1e-5
+
+
+def test_gpu_index_api(cuda, tmp_path):
+    from scarf_amd.knn import GpuIndex
+    rng = np.random.default_rng(9)
+    X = rng.standard_normal((3000, 24)).astype(np.float32)
+    idx = GpuIndex(space="l2", dim=24, screen="fp32")
+    idx.init_index(max_elements=3000, ef_construction=50, M=48, random_seed=4466)
+    idx.set_ef(50)
+    idx.set_num_threads(1)
+    for s in range(0, 3000, 700):
+        idx.add_items(X[s:s + 700])
+    assert idx.get_current_count() == 3000
+    lab, dist = idx.knn_query(X[:100], k=7)
+    assert lab.dtype == np.uint64 and dist.dtype == np.float32
+    ri, rd = oknn.knn_f64(X[:100], X, 7)
+    np.testing.assert_array_equal(lab.astype(np.int64), ri)
+    p = str(tmp_path / "ann_idx")
+    idx.save_index(p)
+    j = GpuIndex(space="l2", dim=24)
+    j.load_index(p)
+    lab2, dist2 = j.knn_query(X[:100], k=7)
+    np.testing.assert_array_equal(lab, lab2)
+    with pytest.raises(RuntimeError):
+        idx.add_items(X[:10])  # exceeds max_elements
+    with pytest.raises(RuntimeError):
+        idx.knn_query(X[:1], k=5000)
+
+
+def test_fix_knn_query_gpu(cuda):
+    from scarf_amd.ann import fix_knn_query
+    rng = np.random.default_rng(2)
+    n, k1 = 400, 12
+    ind = rng.integers(0, 1000, (n, k1)).astype(np.uint64)
+    ref = rng.integers(0, 1000, n).astype(np.int64)
+    where = rng.integers(-1, k1, n)
+    for i, w in enumerate(where):
+        ind[i][ind[i] == ref[i]] = 5000 + i  # make self absent
+        if w >= 0:
+            ind[i, w] = ref[i]
+    dist = rng.random((n, k1)).astype(np.float32)
+    a, b, c = fix_knn_query(ind, dist, ref)
+    ra, rb, rc = oknn.fix_knn_query(ind.astype(np.int64), dist, ref)
+    np.testing.assert_array_equal(a.astype(np.int64), ra)
+    np.testing.assert_array_equal(b, rb)
+    assert c == rc
+
+
+def test_transform_ann_noncontiguous_self(cuda, ref_data):
+    from scarf_amd.datastore import make_graph
+    _, csr = ref_data
+    ann = make_graph(csr, None, dims=10, k=8, batch_size=1000, screen="fp32",
+                     return_ann_object=True)
+    E = ann.annIdx.embedding()
+    rows = np.array([5, 900, 17, 2399, 1200])
+    i, d, nm = ann.transform_ann(E[rows].cpu().numpy(), k=8, self_indices=rows)
+    ki, kd = oknn.knn_f64(E[rows].cpu().numpy(), E.cpu().numpy(), 9)
+    fi, fd, fnm = oknn.fix_knn_query(ki, kd, rows)
+    np.testing.assert_array_equal(i.astype(np.int64), fi)
+    assert nm == fnm
