@@ -28,6 +28,87 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def serving_bench(seconds: float, rate: float = 20.0, n_ref: int = 500_000, d: int = 2000,
+                  seed: int = 1003):
+    """Mapping server on a resident 500k-cell reference: Poisson arrivals, log-uniform
+    request sizes in [1, 50000]; returns latency percentiles and throughput."""
+    import threading
+
+    import numpy as np
+    import torch
+
+    from scarf_amd.device import DeviceCSR
+    from scarf_amd.knn import GpuIndex
+    from scarf_amd.normalize import NormParams, feature_stats
+    from scarf_amd.pca import Projector, fit_pca
+    from scarf_amd.serving import MappingServer
+    from scarf_amd.synth import CSR, generate_csr_gpu, make_gene_model
+
+    model = make_gene_model(d, 20, 0.08, seed=seed)
+    vals, cols, rp = generate_csr_gpu(n_ref, model, seed=seed)
+    ref = DeviceCSR.from_tensors(vals, cols, rp, d)
+    norm = NormParams(size_factor=1e4, clip=10.0)
+    st = feature_stats(ref, 50_000, norm)
+    fit = fit_pca(ref, st, norm, 50, 50_000, mode="exact")
+    mu, sg = st.mu.cpu().numpy(), st.sigma.cpu().numpy()
+    E = Projector.build(fit.loadings, mu, sg, clip=10.0).project(ref, norm, libsize=st.libsize)
+    index = GpuIndex("l2", 50, screen="bf16", pool=64)
+    index.init_index(n_ref)
+    index.add_items(E)
+    index.packed()
+    del ref, vals, cols, rp
+    # query pool: same model with a LogNormal(0, 0.2) per-gene batch shift, sent from host
+    qv, qc, qr = generate_csr_gpu(60_000, model, seed=seed + 500, batch_shift_sigma=0.2)
+    pool = CSR(qv.cpu().numpy().view(np.uint32), qc.cpu().numpy(), qr.cpu().numpy(), d)
+    del qv, qc, qr
+    ids = [f"g{i}" for i in range(d)]
+    rng = np.random.default_rng(seed)
+    n_req = int(seconds * rate)
+    sizes = np.exp(rng.uniform(0, np.log(50_000), n_req)).astype(np.int64).clip(1, 50_000)
+    gaps = rng.exponential(1.0 / rate, n_req)
+    starts = rng.integers(0, pool.n_rows - 50_000, n_req)
+    reqs = [pool.row_slice(int(s), int(s + n)) for s, n in zip(starts, sizes)]
+    one_k = [pool.row_slice(int(s), int(s) + 1000)
+             for s in rng.integers(0, pool.n_rows - 1000, int(5 * rate))]
+    torch.cuda.synchronize()
+    out = {}
+    with MappingServer(fit.loadings, mu, sg, index, ids, ids, norm=norm, k=11,
+                       mem_budget=8 << 30, chunk_rows=16384, n_streams=4) as srv:
+        srv.submit(reqs[0]).result()  # warm the kernels
+        for name, stream, gap in (("mixed", reqs, gaps), ("1k", one_k,
+                                                          rng.exponential(1.0 / rate, len(one_k)))):
+            futs = []
+            t0 = time.perf_counter()
+
+            def client():
+                t = t0
+                for r, g in zip(stream, gap):
+                    t += g
+                    dt = t - time.perf_counter()
+                    if dt > 0:
+                        time.sleep(dt)
+                    futs.append(srv.submit(r))
+
+            th = threading.Thread(target=client)
+            th.start()
+            th.join()
+            res = [f.result() for f in futs]
+            t1 = time.perf_counter()
+            lat = np.array([r.latency_ms for r in res])
+            cells = sum(r.n_cells for r in res)
+            errs = sum(r.error is not None for r in res)
+            out[name] = {"requests": len(res), "errors": errs, "p50_ms": float(np.percentile(lat, 50)),
+                         "p99_ms": float(np.percentile(lat, 99)), "max_ms": float(lat.max()),
+                         "cells": int(cells), "cells_per_s": cells / (t1 - t0)}
+            if name == "mixed":
+                sz = np.array([r.n_cells for r in res])
+                sel = (sz >= 500) & (sz <= 2000)
+                if sel.any():
+                    out[name]["p99_ms_500_2000_cells"] = float(np.percentile(lat[sel], 99))
+            log(f"serving {name}: {out[name]}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -41,6 +122,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=20_000)
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serving-seconds", type=float, default=12.0)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -163,10 +245,22 @@ def main():
         "mapping_p99_ms": None,
     }
 
+    if not args.no_serving:
+        del res
+        torch.cuda.empty_cache()
+        sv = serving_bench(args.serving_seconds)
+        out["mapping_p50_ms"] = sv["mixed"]["p50_ms"]
+        out["mapping_p99_ms"] = sv["mixed"]["p99_ms"]
+        out["mapping_1k_p50_ms"] = sv["1k"]["p50_ms"]
+        out["mapping_1k_p99_ms"] = sv["1k"]["p99_ms"]
+        out["serving"] = {"config": "500000-cell x 2000-gene reference resident (PCA 50, BF16 "
+                                    "index), queries log-uniform 1..50000 cells, Poisson 20/s, "
+                                    "k=11 + mapping scores; '1k' = 1000-cell requests at 20/s",
+                          **sv}
+
     if not args.no_cpu_baseline:
         from oracle.baseline import cpu_pipeline_sample
         ns = min(args.cpu_sample, n)
-        sub = res  # noqa: F841
         # dense sample rows of the same workload (first ns cells)
         rpn = csr.rowptr[: ns + 1].cpu().numpy()
         vv = csr.vals[: int(rpn[-1])].cpu().numpy().view(np.uint32)

@@ -124,6 +124,9 @@ int sh_project(const uint32_t *vals, const int32_t *cols, const int64_t *rowptr,
  *   4. sh_knn_exact      FP64 brute force for rows the certificate rejects (and the
  *                        FP64 reference for recall measurements). */
 int32_t sh_knn_pack_k(int dtype, int32_t D);
+/* Bytes per packed row: K elements (F32 4 B, BF16 2 B), or for FP8 K bytes of E4M3
+ * followed by a 16-byte trailer {f32 ||r||^2, i32 E8M0 scale exponent + 127, pad}. */
+int64_t sh_knn_pack_row_bytes(int dtype, int32_t D);
 int sh_knn_pack(const float *E, int64_t n, int32_t D, int64_t lde, int dtype, int role,
                 void *out, float *aux_out, sh_stream_t stream);
 int sh_knn_screen(const void *qpack, int64_t nq, const void *rpack, int64_t nr,

@@ -42,9 +42,17 @@ COMMON_FLAGS = [
 ]
 
 
+# per-source extra flags: the kNN screen keeps MFMA accumulators in VGPRs (its filter
+# reads them every tile) and assumes no NaNs (INF is its only sentinel)
+EXTRA_FLAGS = {
+    "sh_knn.hip": ["-mllvm", "--amdgpu-mfma-vgpr-form", "-fno-honor-nans"],
+}
+
+
 def _compile(src: str) -> str:
     obj = os.path.join(OBJDIR, os.path.basename(src).replace(".hip", ".o"))
-    cmd = [_hipcc(), *COMMON_FLAGS, "-c", src, "-o", obj]
+    cmd = [_hipcc(), *COMMON_FLAGS, *EXTRA_FLAGS.get(os.path.basename(src), []), "-c", src,
+           "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{r.stderr}")

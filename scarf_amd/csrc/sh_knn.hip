@@ -39,7 +39,77 @@ __device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint
 int32_t pack_k(int dtype, int32_t D) {
   if (dtype == SH_F32) return (int32_t)align_up((size_t)D + 1, 8);
   if (dtype == SH_BF16) return (int32_t)align_up((size_t)D + 3, 16);
+  if (dtype == SH_FP8) return (int32_t)align_up((size_t)D, 64);
   return -1;
+}
+
+// Bytes per packed row: K operand elements, plus (FP8) a 16-byte trailer holding the
+// row's ||r||^2 (f32) and its E8M0 scale exponent (biased, i32).
+int64_t pack_row_bytes(int dtype, int32_t D) {
+  const int32_t K = pack_k(dtype, D);
+  if (K < 0) return -1;
+  if (dtype == SH_F32) return (int64_t)K * 4;
+  if (dtype == SH_BF16) return (int64_t)K * 2;
+  return (int64_t)K + 16;
+}
+
+// float -> OCP FP8 E4M3 (bias 7, max 448, no infinities), round to nearest even,
+// saturating at +-448.
+__device__ __forceinline__ uint8_t f2e4m3(float x) {
+  const uint32_t sign = (__float_as_uint(x) >> 24) & 0x80u;
+  const float a = fabsf(x);
+  if (!(a == a)) return (uint8_t)(sign | 0x7Fu);
+  if (a >= 464.0f) return (uint8_t)(sign | 0x7Eu);
+  if (a < 0.015625f) {  // subnormal: multiples of 2^-9 (8 -> the smallest normal)
+    const int m = (int)rintf(a * 512.0f);
+    return (uint8_t)(sign | (uint32_t)m);
+  }
+  int e;
+  frexpf(a, &e);                              // a = f * 2^e, f in [0.5, 1)
+  int E = e - 1 + 7;                          // biased exponent of 1.m * 2^(e-1)
+  int m = (int)rintf((ldexpf(a, 1 - e) - 1.0f) * 8.0f);
+  if (m == 8) {
+    m = 0;
+    ++E;
+  }
+  if (E > 15 || (E == 15 && m == 7)) return (uint8_t)(sign | 0x7Eu);
+  return (uint8_t)(sign | ((uint32_t)E << 3) | (uint32_t)m);
+}
+
+// FP8 rows: x / 2^e in E4M3 with e the smallest power of two that brings max|x|
+// under 448; role 0 packs r and its ||r||^2, role 1 packs -2q.
+__global__ void k_pack_fp8(const float *__restrict__ E, int64_t n, int32_t D, int64_t lde,
+                           int role, int32_t K, uint8_t *__restrict__ out,
+                           float *__restrict__ aux) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const float *e = E + r * lde;
+  uint8_t *o = out + r * (int64_t)(K + 16);
+  double s = 0.0;
+  float amax = 0.f;
+  for (int t = 0; t < D; ++t) {
+    const float x = e[t];
+    s += (double)x * (double)x;
+    amax = fmaxf(amax, fabsf(role == 0 ? x : 2.0f * x));
+  }
+  int ex = 0;
+  if (amax > 0.f) {
+    ex = (int)ceilf(log2f(amax / 448.0f));
+    while (ldexpf(amax, -ex) > 448.0f) ++ex;
+    ex = max(-126, min(126, ex));
+  }
+  for (int t = 0; t < D; ++t) {
+    const float x = role == 0 ? e[t] : -2.0f * e[t];
+    o[t] = f2e4m3(ldexpf(x, -ex));
+  }
+  for (int t = D; t < K; ++t) o[t] = 0;
+  float *tf = reinterpret_cast<float *>(o + K);
+  int32_t *ti = reinterpret_cast<int32_t *>(o + K);
+  tf[0] = role == 0 ? (float)s : 0.f;
+  ti[1] = ex + 127;  // E8M0 biased exponent
+  ti[2] = 0;
+  ti[3] = 0;
+  if (aux) aux[r] = (float)s;
 }
 
 // ---------------------------------------------------------------------------------
@@ -97,6 +167,7 @@ struct OpF32 {
   typedef float Frag;
   static constexpr int EPL = 1;  // elements per lane per MFMA
   static constexpr int ESZ = 4;
+  static constexpr bool kScaled = false;
   static __device__ __forceinline__ floatx16 mma(Frag a, Frag b, floatx16 c) {
     return __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, c, 0, 0, 0);
   }
@@ -105,15 +176,32 @@ struct OpBF16 {
   typedef bf16x8 Frag;
   static constexpr int EPL = 8;
   static constexpr int ESZ = 2;
+  static constexpr bool kScaled = false;
   static __device__ __forceinline__ floatx16 mma(Frag a, Frag b, floatx16 c) {
     return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+  }
+};
+// FP8 E4M3 with per-row E8M0 scales (MX-style, one scale per lane's 32-element block):
+// the norms enter through the accumulator, read from each row's 16-byte trailer.
+typedef int intx8 __attribute__((ext_vector_type(8)));
+struct OpFP8 {
+  typedef intx8 Frag;
+  static constexpr int EPL = 32;
+  static constexpr int ESZ = 1;
+  static constexpr bool kScaled = true;
+  static __device__ __forceinline__ floatx16 mma_s(Frag a, Frag b, floatx16 c, int sa, int sb) {
+    return __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a, b, c, 0, 0, 0, sa, 0, sb);
+  }
+  static __device__ __forceinline__ floatx16 mma(Frag a, Frag b, floatx16 c) {
+    return mma_s(a, b, c, 127, 127);
   }
 };
 
 constexpr int kQG = 2;                 // 32-query groups per wave
 constexpr int kQPW = 32 * kQG;         // queries per wave
 constexpr int kQPB = 4 * kQPW;         // queries per workgroup
-constexpr int kRT = 64;                // references per LDS stage
+constexpr int kSliceAlign = 128;       // reference slices start on this multiple
+constexpr float kOpen = FLT_MAX;       // threshold of a pool that is not full yet
 
 template <int CAP>
 __device__ __forceinline__ void compact_query(uint64_t *qbuf, int *cnt_slot, float *tau_slot,
@@ -140,10 +228,24 @@ __device__ __forceinline__ void compact_query(uint64_t *qbuf, int *cnt_slot, flo
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
   if (lane == 0) {
     *cnt_slot = keep;
-    *tau_slot = c >= pool ? ord_float((uint32_t)(kp >> 32)) : INFINITY;
+    *tau_slot = c >= pool ? ord_float((uint32_t)(kp >> 32)) : kOpen;
     if (out_cnt) *out_cnt = keep;
   }
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+}
+
+// Appends the values of one 4-reference quad of accumulator group g that pass tau.
+template <int V0>
+__device__ __forceinline__ void insert_quad(const floatx16 &acc, float tau, bool qv, int64_t rb,
+                                           int h, int *cnt, uint64_t *qbuf) {
+#pragma unroll
+  for (int v = V0; v < V0 + 4; ++v) {
+    if (qv && acc[v] <= tau) {
+      const int64_t ref = rb + 8 * (v >> 2) + 4 * h + (v & 3);
+      const int pos = atomicAdd(cnt, 1);
+      qbuf[pos] = make_key(acc[v], (uint32_t)ref);
+    }
+  }
 }
 
 template <class Op, int K, int CAP>
@@ -154,9 +256,17 @@ __global__ void __launch_bounds__(256) k_screen(
   typedef typename Op::Frag Frag;
   constexpr int HALF = K / 2;                 // elements per lane half
   constexpr int NS = HALF / Op::EPL;          // MFMA steps per tile
-  constexpr int ROWB = K * Op::ESZ + 16;      // padded LDS row bytes
-  constexpr int CHUNKS = K * Op::ESZ / 16;    // 16-byte chunks per packed row
-  __shared__ __attribute__((aligned(16))) unsigned char lds_r[kRT * ROWB];
+  constexpr int GROWB = K * Op::ESZ + (Op::kScaled ? 16 : 0);  // packed row bytes in HBM
+  constexpr int ROWB = K * Op::ESZ + 16;      // LDS row bytes (pad or FP8 trailer)
+  constexpr int CHUNKS = GROWB / 16;          // 16-byte chunks per packed row
+  // references per LDS stage: as many as two stage buffers fit in 60 KiB
+  constexpr int RT = (2 * 128 * ROWB <= 61440) ? 128 : (2 * 64 * ROWB <= 61440) ? 64 : 32;
+  // stage buffers are filled by global_load_lds_dwordx4: each wave instruction writes
+  // 64 consecutive 16-byte chunks (1 KiB), so a buffer is a whole number of KiB
+  constexpr int CPR = ROWB / 16;                      // chunks per padded LDS row
+  constexpr int NINSTR = (RT * CPR + 63) / 64;        // wave instructions per stage
+  constexpr int BUFB = NINSTR * 1024;
+  __shared__ __attribute__((aligned(1024))) unsigned char lds_r[2][BUFB];
   __shared__ int cnt_lds[kQPB];
   __shared__ float tau_lds[kQPB];
 
@@ -169,111 +279,152 @@ __global__ void __launch_bounds__(256) k_screen(
   const int64_t qb0 = (int64_t)blockIdx.x * kQPB;
 
   Frag bq[kQG][NS];
+  float tau[kQG];
+  bool qvalid[kQG];
+  int qscale[kQG];
 #pragma unroll
   for (int g = 0; g < kQG; ++g) {
     const int64_t q = qb0 + qw0 + g * 32 + n;
+    qvalid[g] = q < nq;
+    tau[g] = kOpen;
+    qscale[g] = 127;
 #pragma unroll
-    for (int s = 0; s < NS; ++s) {
-      if (q < nq)
-        bq[g][s] = *reinterpret_cast<const Frag *>(Qp + (q * K + h * HALF + s * Op::EPL) * Op::ESZ);
-      else
-        bq[g][s] = Frag{};
-    }
+    for (int s = 0; s < NS; ++s)
+      bq[g][s] = qvalid[g] ? *reinterpret_cast<const Frag *>(
+                                 Qp + q * GROWB + (h * HALF + s * Op::EPL) * Op::ESZ)
+                           : Frag{};
+    if (Op::kScaled && qvalid[g])
+      qscale[g] = *reinterpret_cast<const int32_t *>(Qp + q * GROWB + K * Op::ESZ + 4);
   }
   if (h == 0) {
 #pragma unroll
     for (int g = 0; g < kQG; ++g) {
       cnt_lds[qw0 + g * 32 + n] = 0;
-      tau_lds[qw0 + g * 32 + n] = INFINITY;
+      tau_lds[qw0 + g * 32 + n] = kOpen;
     }
-  }
-  float tau[kQG];
-  bool qvalid[kQG];
-#pragma unroll
-  for (int g = 0; g < kQG; ++g) {
-    tau[g] = INFINITY;
-    qvalid[g] = qb0 + qw0 + g * 32 + n < nq;
   }
 
-  for (int64_t t0 = r_begin; t0 < r_end; t0 += kRT) {
-    __syncthreads();
-    for (int e = threadIdx.x; e < kRT * CHUNKS; e += 256) {
-      const int rr = e / CHUNKS, c = e % CHUNKS;
-      const int64_t r = t0 + rr;
-      uint4 v = make_uint4(0, 0, 0, 0);
-      if (r < r_end) v = *reinterpret_cast<const uint4 *>(Rp + (r * K) * Op::ESZ + c * 16);
-      *reinterpret_cast<uint4 *>(lds_r + rr * ROWB + c * 16) = v;
+  // double buffering: stage t+1 streams into the other buffer (LDS DMA) while stage t
+  // computes.  Chunk ci of a stage lands at byte 16*ci: row ci / CPR, column chunk
+  // ci % CPR; the padding chunk of each row and rows past the slice re-read a valid
+  // address (their values are never used).
+#define SH_SCREEN_FETCH(T0, B)                                                           \
+  _Pragma("unroll") for (int j = w; j < NINSTR; j += 4) {                                \
+    const int ci = 64 * j + lane;                                                        \
+    const int rr = ci / CPR, c = ci % CPR;                                               \
+    int64_t r = (T0) + rr;                                                               \
+    r = r < r_end ? r : r_end - 1;                                                       \
+    const unsigned char *src = Rp + r * GROWB + (c < CHUNKS ? c : 0) * 16;               \
+    __builtin_amdgcn_global_load_lds(                                                    \
+        (const void *)src,                                                               \
+        (__attribute__((address_space(3))) void *)(&lds_r[(B)][1024 * j]), 16, 0, 0);    \
+  }
+
+  int b = 0;
+  if (r_begin < r_end) {
+    SH_SCREEN_FETCH(r_begin, 0)
+  }
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  for (int64_t t0 = r_begin; t0 < r_end; t0 += RT) {
+    const int64_t tn = t0 + RT;
+    if (tn < r_end) {
+      SH_SCREEN_FETCH(tn, b ^ 1)
     }
-    __syncthreads();
 #pragma unroll
-    for (int sub = 0; sub < kRT / 32; ++sub) {
+    for (int sub = 0; sub < RT / 32; ++sub) {
       const int64_t rb = t0 + sub * 32;
-      if (rb >= r_end) break;
+      if (rb >= r_end) continue;
       floatx16 acc[kQG];
+      if constexpr (Op::kScaled) {
+        // accumulator starts at ||r||^2 of each of the lane's 16 references
+        floatx16 nrm;
 #pragma unroll
-      for (int g = 0; g < kQG; ++g)
+        for (int v = 0; v < 16; ++v)
+          nrm[v] = *reinterpret_cast<const float *>(
+              &lds_r[b][(sub * 32 + 8 * (v >> 2) + 4 * h + (v & 3)) * ROWB + K * Op::ESZ]);
 #pragma unroll
-        for (int v = 0; v < 16; ++v) acc[g][v] = 0.f;
-      const unsigned char *arow = lds_r + (sub * 32 + n) * ROWB + h * HALF * Op::ESZ;
+        for (int g = 0; g < kQG; ++g) acc[g] = nrm;
+      } else {
+#pragma unroll
+        for (int g = 0; g < kQG; ++g)
+#pragma unroll
+          for (int v = 0; v < 16; ++v) acc[g][v] = 0.f;
+      }
+      const unsigned char *arow = &lds_r[b][(sub * 32 + n) * ROWB + h * HALF * Op::ESZ];
+      int ascale = 127;
+      if constexpr (Op::kScaled)
+        ascale = *reinterpret_cast<const int32_t *>(&lds_r[b][(sub * 32 + n) * ROWB + K * Op::ESZ + 4]);
 #pragma unroll
       for (int s = 0; s < NS; ++s) {
         Frag a = *reinterpret_cast<const Frag *>(arow + s * Op::EPL * Op::ESZ);
 #pragma unroll
-        for (int g = 0; g < kQG; ++g) acc[g] = Op::mma(a, bq[g][s], acc[g]);
+        for (int g = 0; g < kQG; ++g) {
+          if constexpr (Op::kScaled)
+            acc[g] = Op::mma_s(a, bq[g][s], acc[g], ascale, qscale[g]);
+          else
+            acc[g] = Op::mma(a, bq[g][s], acc[g]);
+        }
       }
-      if (rb + 32 > r_end) {  // references past the slice end: NaN never passes
+      if (rb + 32 > r_end) {  // references past the slice end never pass
 #pragma unroll
         for (int g = 0; g < kQG; ++g)
 #pragma unroll
           for (int v = 0; v < 16; ++v) {
             const int64_t ref = rb + 8 * (v >> 2) + 4 * h + (v & 3);
-            if (ref >= r_end) acc[g][v] = __builtin_nanf("");
+            if (ref >= r_end) acc[g][v] = INFINITY;
           }
       }
-      bool hit = false;
+      // quad minima, then the tile minimum, per query group
+      float m4[kQG][4];
+      bool hit[kQG];
+      bool any_hit = false;
 #pragma unroll
       for (int g = 0; g < kQG; ++g) {
-        float m = acc[g][0];
 #pragma unroll
-        for (int v = 1; v < 16; ++v) m = fminf(m, acc[g][v]);
-        hit |= qvalid[g] && (m <= tau[g]);
+        for (int j = 0; j < 4; ++j)
+          m4[g][j] = fminf(fminf(acc[g][4 * j], acc[g][4 * j + 1]),
+                           fminf(acc[g][4 * j + 2], acc[g][4 * j + 3]));
+        const float m = fminf(fminf(m4[g][0], m4[g][1]), fminf(m4[g][2], m4[g][3]));
+        hit[g] = qvalid[g] && m <= tau[g];
+        any_hit |= hit[g];
       }
-      if (__any(hit)) {
+      if (__any(any_hit)) {
 #pragma unroll
         for (int g = 0; g < kQG; ++g) {
+          if (!__any(hit[g])) continue;
           const int ql = qw0 + g * 32 + n;
           const int64_t q = qb0 + ql;
-          uint64_t *qbuf = buf + ((q < nq ? q : 0) * S + slice) * CAP;
-#pragma unroll
-          for (int v = 0; v < 16; ++v) {
-            if (q < nq && acc[g][v] <= tau[g]) {
-              const int64_t ref = rb + 8 * (v >> 2) + 4 * h + (v & 3);
-              const int pos = atomicAdd(&cnt_lds[ql], 1);
-              qbuf[pos] = make_key(acc[g][v], (uint32_t)ref);
-            }
-          }
-        }
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-        // compact every query of this wave whose buffer cannot take another tile
-#pragma unroll
-        for (int g = 0; g < kQG; ++g) {
-          const int ql = qw0 + g * 32 + n;
-          bool need = cnt_lds[ql] > CAP - 32;
-          uint64_t m = __ballot(need);
-          while (m) {
-            const int src = __ffsll((unsigned long long)m) - 1;
-            m &= m - 1;
-            if (src >= 32) continue;  // lanes n and n+32 share a query
+          uint64_t *qbuf = buf + ((qvalid[g] ? q : 0) * S + slice) * CAP;
+          if (__any(hit[g] && m4[g][0] <= tau[g]))
+            insert_quad<0>(acc[g], tau[g], qvalid[g], rb, h, &cnt_lds[ql], qbuf);
+          if (__any(hit[g] && m4[g][1] <= tau[g]))
+            insert_quad<4>(acc[g], tau[g], qvalid[g], rb, h, &cnt_lds[ql], qbuf);
+          if (__any(hit[g] && m4[g][2] <= tau[g]))
+            insert_quad<8>(acc[g], tau[g], qvalid[g], rb, h, &cnt_lds[ql], qbuf);
+          if (__any(hit[g] && m4[g][3] <= tau[g]))
+            insert_quad<12>(acc[g], tau[g], qvalid[g], rb, h, &cnt_lds[ql], qbuf);
+          __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+          // compact every query of this group whose buffer cannot take another tile
+          const bool need = cnt_lds[ql] > CAP - 32;
+          uint64_t mk = __ballot(need) & 0xffffffffull;  // lanes n and n+32 share a query
+          while (mk) {
+            const int src = __ffsll((unsigned long long)mk) - 1;
+            mk &= mk - 1;
             const int qls = qw0 + g * 32 + src;
             const int64_t qs = qb0 + qls;
-            compact_query<CAP>(buf + (qs * S + slice) * CAP, &cnt_lds[qls], &tau_lds[qls], pool,
-                               nullptr, nullptr);
+            compact_query<CAP>(buf + (qs * S + slice) * CAP, &cnt_lds[qls], &tau_lds[qls],
+                               pool, nullptr, nullptr);
           }
           tau[g] = tau_lds[ql];
         }
       }
     }
+    __builtin_amdgcn_s_waitcnt(0);  // the next stage's DMA has landed
+    __syncthreads();
+    b ^= 1;
   }
+#undef SH_SCREEN_FETCH
   // final: sort each query's buffer and emit its best `pool`
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
   for (int j = 0; j < kQPW; ++j) {
@@ -285,7 +436,6 @@ __global__ void __launch_bounds__(256) k_screen(
   }
 }
 
-// ---------------------------------------------------------------------------------
 // Exact FP64 squared distance, terms added in feature order, no contraction: the
 // same evaluation as the CPU oracle, so results agree bit for bit.
 __device__ __forceinline__ double sqdist_exact(const float *__restrict__ q_lds,
@@ -606,6 +756,7 @@ extern "C" int sh_fix_knn_query(const int64_t *idx_in, const float *dist_in,
 }
 
 extern "C" int32_t sh_knn_pack_k(int dtype, int32_t D) { return pack_k(dtype, D); }
+extern "C" int64_t sh_knn_pack_row_bytes(int dtype, int32_t D) { return pack_row_bytes(dtype, D); }
 
 extern "C" int sh_knn_pack(const float *E, int64_t n, int32_t D, int64_t lde, int dtype, int role,
                            void *out, float *aux_out, sh_stream_t stream) {
@@ -623,9 +774,12 @@ extern "C" int sh_knn_pack(const float *E, int64_t n, int32_t D, int64_t lde, in
   if (dtype == SH_F32)
     hipLaunchKernelGGL(k_pack_f32, grid, block, 0, (hipStream_t)stream, E, n, D, lde, role, K,
                        (float *)out, aux_out);
-  else
+  else if (dtype == SH_BF16)
     hipLaunchKernelGGL(k_pack_bf16, grid, block, 0, (hipStream_t)stream, E, n, D, lde, role, K,
                        (uint16_t *)out, aux_out);
+  else
+    hipLaunchKernelGGL(k_pack_fp8, grid, block, 0, (hipStream_t)stream, E, n, D, lde, role, K,
+                       (uint8_t *)out, aux_out);
   SH_LAUNCH_CHECK("k_pack");
   return SH_OK;
 }
@@ -665,10 +819,10 @@ extern "C" int sh_knn_screen(const void *qpack, int64_t nq, const void *rpack, i
   }
   if (nq == 0) return SH_OK;
   const int32_t K = pack_k(dtype, D);
-  int64_t slice_len = align_up((size_t)cdiv(nr, n_slices), kRT);
+  int64_t slice_len = align_up((size_t)cdiv(nr, n_slices), kSliceAlign);
   const int S = (int)cdiv(nr, slice_len);
   SH_ARG(S == n_slices, "sh_knn_screen: %d slices requested but %lld references give %d; use "
-         "a slice count whose slices hold >= 64 references", n_slices, (long long)nr, S);
+         "a slice count whose slices hold >= 128 references", n_slices, (long long)nr, S);
   dim3 grid(cdiv(nq, kQPB), S);
   hipStream_t st = (hipStream_t)stream;
   uint64_t *buf = (uint64_t *)workspace;
@@ -693,6 +847,9 @@ extern "C" int sh_knn_screen(const void *qpack, int64_t nq, const void *rpack, i
     SH_SCREEN_CASE(OpBF16, 96)
     SH_SCREEN_CASE(OpBF16, 112)
     SH_SCREEN_CASE(OpBF16, 128)
+  } else if (dtype == SH_FP8) {
+    SH_SCREEN_CASE(OpFP8, 64)
+    SH_SCREEN_CASE(OpFP8, 128)
   } else if (dtype == SH_F32) {
     SH_SCREEN_CASE(OpF32, 8)
     SH_SCREEN_CASE(OpF32, 16)

@@ -85,10 +85,10 @@ def pack(E: "torch.Tensor", dtype: str, role: int, stream=None):
     code = _lib.DTYPES[dtype]
     n, D = E.shape
     K = int(_lib.lib().sh_knn_pack_k(code, D))
-    if K <= 0:
+    rowb = int(_lib.lib().sh_knn_pack_row_bytes(code, D))
+    if K <= 0 or rowb <= 0:
         raise ValueError(f"screen dtype {dtype} unsupported for D={D}")
-    tdt = {"fp32": torch.float32, "f32": torch.float32, "bf16": torch.int16}[dtype]
-    out = torch.empty((max(n, 1), K), dtype=tdt, device=E.device)
+    out = torch.empty((max(n, 1), rowb), dtype=torch.uint8, device=E.device)
     aux = torch.empty(max(n, 1), dtype=torch.float32, device=E.device)
     _lib.call("sh_knn_pack", ptr(E), n, D, E.stride(0), code, role, ptr(out), ptr(aux),
               stream_handle(stream))
@@ -110,9 +110,9 @@ def default_slices(nq: int, nr: int) -> int:
 
 
 def _fix_slices(nr: int, s: int) -> int:
-    # the kernel requires every slice to hold >= 1 reference after 64-alignment
+    # the kernel requires every slice to hold >= 1 reference after 128-alignment
     while s > 1:
-        sl = int(math.ceil(math.ceil(nr / s) / 64) * 64)
+        sl = int(math.ceil(math.ceil(nr / s) / 128) * 128)
         if math.ceil(nr / sl) == s:
             return s
         s -= 1

@@ -115,5 +115,5 @@ def test_slice_plan():
     for nr in (64, 100, 5000, 500_000, 1_000_000):
         for nq in (1, 100, 1000, 50_000, 1_000_000):
             s = _fix_slices(nr, default_slices(nq, nr))
-            sl = -(-(-(-nr // s)) // 64) * 64
+            sl = -(-(-(-nr // s)) // 128) * 128
             assert -(-nr // sl) == s

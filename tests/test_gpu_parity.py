@@ -264,3 +264,70 @@ def test_mapping_scores_gpu(cuda):
     for (g1, m1), (g2, m2) in zip(got, want):
         assert g1 == g2
         np.testing.assert_allclose(m1, m2, rtol=1e-11, atol=1e-12)
+
+
+def _e4m3_table():
+    vals = []
+    for code in range(256):
+        s, e, m = code >> 7, (code >> 3) & 15, code & 7
+        if e == 15 and m == 7:
+            continue  # NaN
+        v = (m / 8.0) * 2.0 ** -6 if e == 0 else (1 + m / 8.0) * 2.0 ** (e - 7)
+        vals.append((-v if s else v, code))
+    return vals
+
+
+def _e4m3_rne(x):
+    """Nearest E4M3 code (ties to even mantissa), saturating at +-448."""
+    tab = _e4m3_table()
+    v = np.array([t[0] for t in tab])
+    c = np.array([t[1] for t in tab])
+    out = np.empty(x.shape, dtype=np.uint8)
+    for i, xi in enumerate(x.ravel()):
+        xi = min(max(xi, -448.0), 448.0)
+        dist = np.abs(v - xi)
+        best = np.where(dist == dist.min())[0]
+        cand = c[best]
+        if len(cand) > 1:
+            even = cand[(cand & 1) == 0]
+            cand = even if len(even) else cand
+            # +0 and -0 tie at 0: take +0
+            cand = np.sort(cand)
+        out.flat[i] = cand[0]
+    return out
+
+
+def test_fp8_pack_rounding(cuda):
+    """The FP8 packer's E4M3 codes and E8M0 scales equal a round-to-nearest-even
+    quantiser applied to x / 2^e."""
+    import torch
+    from scarf_amd.knn import pack
+    rng = np.random.default_rng(11)
+    E = (rng.standard_normal((40, 50)) * rng.lognormal(0, 2, (40, 1))).astype(np.float32)
+    E[3] = 0.0
+    E[5, :7] = [448.0, 1e-3, -2e-4, 0.0625, 3.0, -100.0, 1.0 / 1024]
+    out, _, K = pack(torch.from_numpy(E).cuda(), "fp8", 0)
+    b = out.cpu().numpy()
+    for r in range(E.shape[0]):
+        ex = int(b[r, K + 4:K + 8].view(np.int32)[0]) - 127
+        want = _e4m3_rne(np.ldexp(E[r].astype(np.float64), -ex))
+        assert np.abs(E[r]).max() == 0 or np.abs(np.ldexp(E[r], -ex)).max() <= 448
+        np.testing.assert_array_equal(b[r, :50], want)
+        assert b[r, K:K + 4].view(np.float32)[0] == np.float32((E[r].astype(np.float64) ** 2).sum())
+
+
+@pytest.mark.parametrize("D,pool", [(100, 128), (50, 64)])
+def test_knn_fp8_recall(cuda, D, pool):
+    import torch
+    from scarf_amd.knn import pack_refs, search
+    R = _rand_embedding(20000, D, 12)
+    Q = R[:2000]
+    Rt, Qt = torch.from_numpy(R).cuda(), torch.from_numpy(Q).cuda()
+    say("knn fp8 D", D, "pool", pool)
+    refs = pack_refs(Rt, "fp8")
+    idx, dist, _ = search(Qt, Rt, refs, 11, pool=pool)
+    ri, rd = oknn.knn_f64(Q, R, 11)
+    got = idx.cpu().numpy()
+    recall = np.mean([len(set(got[i]) & set(ri[i])) / 11 for i in range(len(Q))])
+    say("fp8 recall@11", recall)
+    assert recall >= 0.999
