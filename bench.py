@@ -1,17 +1,19 @@
 #!/usr/bin/env python
 """Throughput of scarf's graph-building path on one MI355X.
 
-One step = the whole hot path over the resident 1,000,000-cell x 2,000-gene CSR count
-matrix: library-size normalisation + log1p + per-feature stats, PCA fit (50 dims),
-sparse z-score projection, self kNN (k = 21, BF16 MFMA screen, top-64 pool re-ranked
-exactly), and smooth_knn_dist edge weights.  Counts are synthetic (negative-binomial,
-20 latent groups, ~8% non-zero) drawn on the GPU and resident in HBM before timing.
+Default workload (the metric's): one step = the whole hot path over a resident
+1,000,000-cell x 2,000-gene CSR count matrix -- library-size normalisation + log1p +
+per-feature stats, PCA fit (50 dims), sparse z-score projection, self kNN (k = 21, BF16
+MFMA screen, top-64 pool re-ranked exactly), smooth_knn_dist edge weights -- followed
+by the mapping-server measurement (p50/p99).  Counts are synthetic negative-binomial
+(20 latent groups, ~8% non-zero) drawn on the GPU and resident in HBM before timing.
 
-Prints one JSON line on stdout; progress goes to stderr.
+``--workload`` selects the other configurations: small (5k x 2k, plain FP32 exact kNN),
+100k (IPCA in scarf's block order, BF16 vs FP32 screen), 4m (4M x 3k, PCA 100, FP8
+screen, top-128 pool).  Prints one JSON line on stdout; progress goes to stderr.
 """
 import argparse
 import json
-import math
 import os
 import sys
 import time
@@ -22,6 +24,18 @@ if ROOT not in sys.path:
 
 METRIC = "cells/s norm+PCA+kNN @1M cells, recall@11 vs FP64 oracle; mapping p50/p99 ms"
 PEAK = {"bf16": 2.5e15, "fp32": 157.3e12, "fp8": 5.0e15, "fp64": 78.6e12}
+
+WORKLOADS = {
+    # cells, genes, groups, density, chunk, dims, k, screen, pool, pca mode, recall queries
+    "small": dict(n=5_000, d=2000, groups=1, density=0.10, chunk=5_000, dims=50, k=11,
+                  screen="fp32", pool=64, pca="ipca", recall_q=5_000, seed=1000),
+    "100k": dict(n=100_000, d=2000, groups=8, density=0.10, chunk=20_000, dims=50, k=11,
+                 screen="bf16", pool=64, pca="ipca", recall_q=100_000, seed=1001),
+    "1m": dict(n=1_000_000, d=2000, groups=20, density=0.08, chunk=50_000, dims=50, k=21,
+               screen="bf16", pool=64, pca="exact", recall_q=10_000, seed=1002),
+    "4m": dict(n=4_000_000, d=3000, groups=40, density=0.06, chunk=100_000, dims=100, k=11,
+               screen="fp8", pool=128, pca="exact", recall_q=20_000, seed=1004),
+}
 
 
 def log(*a):
@@ -75,8 +89,8 @@ def serving_bench(seconds: float, rate: float = 20.0, n_ref: int = 500_000, d: i
     with MappingServer(fit.loadings, mu, sg, index, ids, ids, norm=norm, k=11,
                        mem_budget=8 << 30, chunk_rows=16384, n_streams=4) as srv:
         srv.submit(reqs[0]).result()  # warm the kernels
-        for name, stream, gap in (("mixed", reqs, gaps), ("1k", one_k,
-                                                          rng.exponential(1.0 / rate, len(one_k)))):
+        for name, stream, gap in (("mixed", reqs, gaps),
+                                  ("1k", one_k, rng.exponential(1.0 / rate, len(one_k)))):
             futs = []
             t0 = time.perf_counter()
 
@@ -97,7 +111,8 @@ def serving_bench(seconds: float, rate: float = 20.0, n_ref: int = 500_000, d: i
             lat = np.array([r.latency_ms for r in res])
             cells = sum(r.n_cells for r in res)
             errs = sum(r.error is not None for r in res)
-            out[name] = {"requests": len(res), "errors": errs, "p50_ms": float(np.percentile(lat, 50)),
+            out[name] = {"requests": len(res), "errors": errs,
+                         "p50_ms": float(np.percentile(lat, 50)),
                          "p99_ms": float(np.percentile(lat, 99)), "max_ms": float(lat.max()),
                          "cells": int(cells), "cells_per_s": cells / (t1 - t0)}
             if name == "mixed":
@@ -109,16 +124,45 @@ def serving_bench(seconds: float, rate: float = 20.0, n_ref: int = 500_000, d: i
     return out
 
 
+def recall_vs_exact(res, k, n, n_queries, seed=7):
+    """Recall@k (and @11) and k-th distance error of the graph's self-kNN against the
+    exact FP64 top-k (certified FP32 screen + FP64 re-rank, bit-equal to brute force)."""
+    import numpy as np
+    import torch
+
+    from scarf_amd import knn as gknn
+    E = res.embedding
+    rng = np.random.default_rng(seed)
+    qs = np.arange(n) if n_queries >= n else np.sort(rng.choice(n, n_queries, replace=False))
+    qrows = torch.from_numpy(qs).cuda()
+    ex_refs = gknn.pack_refs(E, "fp32")
+    ex_idx, ex_dist, _ = gknn.search(E[qrows].contiguous(), E, ex_refs, k, pool=max(64, k + 1),
+                                     self_idx=qrows)
+    ex_i = ex_idx.cpu().numpy()
+    ex_d = ex_dist.cpu().numpy().astype(np.float64)
+    got = res.indices[qrows].cpu().numpy()
+    gd = res.distances[qrows].cpu().numpy().astype(np.float64)
+    rk = float(np.mean([len(set(got[i]) & set(ex_i[i])) / k for i in range(len(qs))]))
+    k11 = min(11, k)
+    r11 = float(np.mean([len(set(got[i, :k11]) & set(ex_i[i, :k11])) / k11
+                         for i in range(len(qs))]))
+    err = np.abs(gd[:, -1] - ex_d[:, -1]) / np.maximum(ex_d[:, -1], 1e-30)
+    del ex_refs
+    return {"recall_at_k": rk, "recall_at_11": r11, "kth_dist_rel_err_max": float(err.max()),
+            "kth_dist_rel_err_mean": float(err.mean()), "recall_queries": int(len(qs))}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--cells", type=int, default=1_000_000)
-    ap.add_argument("--genes", type=int, default=2000)
-    ap.add_argument("--pca-mode", default="exact", choices=["exact", "ipca"])
-    ap.add_argument("--screen", default="bf16", choices=["bf16", "fp32"])
-    ap.add_argument("--recall-queries", type=int, default=10_000)
+    ap.add_argument("--workload", default="1m", choices=sorted(WORKLOADS))
+    ap.add_argument("--pca-mode", default=None, choices=["exact", "ipca"])
+    ap.add_argument("--screen", default=None, choices=["bf16", "fp32", "fp8"])
+    ap.add_argument("--recall-queries", type=int, default=None)
+    ap.add_argument("--compare-screen", default=None, choices=["bf16", "fp32", "fp8"],
+                    help="also time the kNN stage with this screen on the same embedding")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=20_000)
     ap.add_argument("--no-serving", action="store_true")
@@ -138,17 +182,27 @@ def main():
     from scarf_amd.graph import GraphParams, StageTimer, build_graph
     from scarf_amd.synth import generate_csr_gpu, make_gene_model
 
+    wl = dict(WORKLOADS[args.workload])
+    if args.pca_mode:
+        wl["pca"] = args.pca_mode
+    if args.screen:
+        wl["screen"] = args.screen
+    if args.recall_queries:
+        wl["recall_q"] = args.recall_queries
     torch.cuda.set_device(0)
-    n, d = args.cells, args.genes
+    n, d = wl["n"], wl["d"]
     t0 = time.time()
-    model = make_gene_model(d, 20, 0.08, seed=1002)
-    vals, cols, rp = generate_csr_gpu(n, model, seed=1002)
+    model = make_gene_model(d, wl["groups"], wl["density"], seed=wl["seed"])
+    vals, cols, rp = generate_csr_gpu(n, model, seed=wl["seed"],
+                                      block_rows=min(50_000, max(1, 2_000_000_000 // (4 * d * 8))))
     csr = DeviceCSR.from_tensors(vals, cols, rp, d)
+    del vals, cols, rp
     torch.cuda.synchronize()
+    torch.cuda.empty_cache()
     log(f"generated {n} x {d} CSR, nnz={csr.nnz} ({csr.nnz / (n * d):.3f}) in {time.time() - t0:.1f}s")
 
-    params = GraphParams(dims=50, k=21, batch_size=50_000, size_factor=1e4, clip=10.0,
-                         pca_mode=args.pca_mode, screen=args.screen, pool=64)
+    params = GraphParams(dims=wl["dims"], k=wl["k"], batch_size=wl["chunk"], size_factor=1e4,
+                         clip=10.0, pca_mode=wl["pca"], screen=wl["screen"], pool=wl["pool"])
     for w in range(args.warmup):
         build_graph(csr, params)
         torch.cuda.synchronize()
@@ -163,7 +217,7 @@ def main():
         timer = StageTimer(True)
         res = build_graph(csr, params, timer=timer)
         stage_ms.append(timer.result())
-        log(f"step {s + 1}/{args.steps}: {stage_ms[-1].get('wall_total', 0):.1f} ms")
+        log(f"step {s + 1}/{args.steps}: {stage_ms[-1].get('total', 0):.1f} ms")
     torch.cuda.synchronize()
     t_end = time.perf_counter()
     gknn.disable_kernel_timing()
@@ -177,44 +231,19 @@ def main():
     scr_flops = scr[0][2] if scr else 0.0
     avg_ms = sum(scr_ms) / max(len(scr_ms), 1)
     achieved = scr_flops / (avg_ms * 1e-3) if avg_ms > 0 else 0.0
-    peak = PEAK[args.screen]
-    roofline = {"kernel": "k_screen<OpBF16,64,256>" if args.screen == "bf16" else "k_screen<OpF32,56,256>",
-                "bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12,
-                "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
-                "avg_launch_ms": avg_ms, "launches": len(scr_ms),
-                "work": "2*nq*nr*D flops, D=50 (unpadded), nq=nr=%d" % n}
+    peak = PEAK[wl["screen"]]
+    K = {"bf16": 64, "fp32": 56, "fp8": 64}[wl["screen"]] if wl["dims"] <= 50 else \
+        {"bf16": 112, "fp32": 104, "fp8": 128}[wl["screen"]]
+    cap = 256 if wl["pool"] <= 64 else 512
+    op = {"bf16": "OpBF16", "fp32": "OpF32", "fp8": "OpFP8"}[wl["screen"]]
+    roofline = {"kernel": f"k_screen<{op},{K},{cap}>", "bound": "mfma",
+                "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
+                "frac": achieved / peak, "traffic": None, "avg_launch_ms": avg_ms,
+                "launches": len(scr_ms),
+                "work": f"2*nq*nr*D flops, D={wl['dims']} (unpadded), nq=nr={n}"}
 
-    # recall vs the exact FP64 top-k on sampled queries (certified exact path)
-    rng = np.random.default_rng(7)
-    qs = np.sort(rng.choice(n, min(args.recall_queries, n), replace=False))
-    E = res.embedding
-    qrows = torch.from_numpy(qs).cuda()
-    ex_refs = gknn.pack_refs(E, "fp32")
-    ex_idx, ex_dist, _ = gknn.search(E[qrows].contiguous(), E, ex_refs, params.k + 1, pool=64)
-    ex_idx = ex_idx.cpu().numpy()
-    ex_dist = ex_dist.cpu().numpy()
-    got = res.indices[qrows].cpu().numpy()
-    gd = res.distances[qrows].cpu().numpy()
-    # exact self-query lists: drop the query itself the way fix_knn_query does
-    ex_i, ex_d = [], []
-    for r, q in enumerate(qs):
-        row, dr = list(ex_idx[r]), list(ex_dist[r])
-        if q in row:
-            p = row.index(q)
-            row.pop(p)
-            dr.pop(p)
-        else:
-            row.pop()
-            dr.pop()
-        ex_i.append(row)
-        ex_d.append(dr)
-    ex_i = np.array(ex_i)
-    ex_d = np.array(ex_d)
-    k = params.k
-    rec_k = float(np.mean([len(set(got[i]) & set(ex_i[i])) / k for i in range(len(qs))]))
-    rec_11 = float(np.mean([len(set(got[i, :11]) & set(ex_i[i, :11])) / 11 for i in range(len(qs))]))
-    kth_err = np.abs(gd[:, -1] - ex_d[:, -1]) / np.maximum(ex_d[:, -1], 1e-30)
-    log(f"recall@{k} {rec_k:.5f} recall@11 {rec_11:.5f} kth-dist rel err max {kth_err.max():.2e}")
+    rec = recall_vs_exact(res, wl["k"], n, wl["recall_q"])
+    log(f"recall {rec}")
 
     out = {
         "metric": METRIC,
@@ -227,17 +256,19 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": args.screen,
-        "data": "synthetic NB counts (20 groups, ~8% nnz) drawn on the GPU, CSR resident in HBM",
-        "config": {"workload": f"{n} cells x {d} genes CSR u32, chunk 50000, PCA 50 "
-                               f"({args.pca_mode}, FP64 Gram + FP64 eigensolve), k=21, "
-                               f"{args.screen} screen top-64 pool + exact FP64 re-rank, "
-                               f"sf=1e4, z clip 10",
+        "dtype": wl["screen"],
+        "data": f"synthetic NB counts ({wl['groups']} groups, ~{wl['density']:.0%} nnz) drawn "
+                f"on the GPU, CSR resident in HBM",
+        "config": {"workload": f"{args.workload}: {n} cells x {d} genes CSR u32, chunk "
+                               f"{wl['chunk']}, PCA {wl['dims']} ({wl['pca']}, FP64 Gram + FP64 "
+                               f"eigensolve), k={wl['k']}, {wl['screen']} screen top-{wl['pool']}"
+                               f" pool + exact FP64 re-rank, sf=1e4, z clip 10",
                    "nnz": csr.nnz},
-        "recall_at_21": rec_k,
-        "recall_at_11": rec_11,
-        "kth_dist_rel_err_max": float(kth_err.max()),
-        "kth_dist_rel_err_mean": float(kth_err.mean()),
+        f"recall_at_{wl['k']}": rec["recall_at_k"],
+        "recall_at_11": rec["recall_at_11"],
+        "kth_dist_rel_err_max": rec["kth_dist_rel_err_max"],
+        "kth_dist_rel_err_mean": rec["kth_dist_rel_err_mean"],
+        "recall_queries": rec["recall_queries"],
         "ann_recall_pct": res.recall,
         "stage_ms": {kk: round(v, 3) for kk, v in stage_ms[-1].items()},
         "roofline": roofline,
@@ -245,7 +276,22 @@ def main():
         "mapping_p99_ms": None,
     }
 
-    if not args.no_serving:
+    if args.compare_screen:
+        alt = gknn.pack_refs(res.embedding, args.compare_screen)
+        E = res.embedding
+        for rep in range(2):
+            torch.cuda.synchronize()
+            ta = time.perf_counter()
+            gknn.search(E, E, alt, wl["k"], wl["pool"], self_offset=0)
+            torch.cuda.synchronize()
+            tb = time.perf_counter()
+        out["compare_screen"] = {"screen": args.compare_screen, "knn_ms": 1e3 * (tb - ta),
+                                 "main_knn_ms": stage_ms[-1].get("knn"),
+                                 "speedup_of_main": 1e3 * (tb - ta) / stage_ms[-1].get("knn")}
+        log(f"compare: {out['compare_screen']}")
+        del alt
+
+    if not args.no_serving and args.workload == "1m":
         del res
         torch.cuda.empty_cache()
         sv = serving_bench(args.serving_seconds)
@@ -261,21 +307,20 @@ def main():
     if not args.no_cpu_baseline:
         from oracle.baseline import cpu_pipeline_sample
         ns = min(args.cpu_sample, n)
-        # dense sample rows of the same workload (first ns cells)
         rpn = csr.rowptr[: ns + 1].cpu().numpy()
         vv = csr.vals[: int(rpn[-1])].cpu().numpy().view(np.uint32)
         cc = csr.cols[: int(rpn[-1])].cpu().numpy()
         dense = np.zeros((ns, d), dtype=np.uint32)
         dense[np.repeat(np.arange(ns), np.diff(rpn)), cc] = vv
         log(f"cpu baseline on {ns} cells ...")
-        cb = cpu_pipeline_sample(dense, 50, 21, 50_000, 1e4, 10.0, n)
+        cb = cpu_pipeline_sample(dense, wl["dims"], wl["k"], wl["chunk"], 1e4, 10.0, n)
         out["cpu_baseline"] = {
             "value": cb["cells_per_s"], "unit": "cells/s", "cores": cb["cores"],
             "kind": "port",
             "sample": f"first {ns} cells of the same matrix: numpy normalise + FP64 stats, "
-                      f"sklearn IncrementalPCA(51) in scarf order, z@W, exact BLAS kNN for "
-                      f"{cb['knn_queries']} queries (hnswlib absent), vectorised smoothing; "
-                      f"each stage scaled to {n} cells (kNN by queries x refs)",
+                      f"sklearn IncrementalPCA({wl['dims'] + 1}) in scarf order, z@W, exact BLAS "
+                      f"kNN for {cb['knn_queries']} queries (hnswlib absent), vectorised "
+                      f"smoothing; each stage scaled to {n} cells (kNN by queries x refs)",
             "stage_seconds_at_full_size": {kk: round(v, 2) for kk, v in cb["est_seconds"].items()},
         }
         out["gpu_vs_cpu"] = value / cb["cells_per_s"]

@@ -55,17 +55,20 @@ const char *sh_last_error(void);
 /* ---- normalisation and feature statistics ------------------------------------
  * Per-cell library size over the mapped columns (0 -> 1) unless `libsize_in` is
  * given, x = log1p(sf * c / libsize) (or sf * c / libsize when log_transform == 0),
- * and the chunk's per-feature FP64 sums of x and x^2 (implicit zeros contribute 0).
- * Deterministic: fixed-order accumulation.
+ * and per-chunk FP64 sums of x and x^2 for every feature (implicit zeros add 0):
+ * rows [0, n_rows) are split into chunks of `chunk_rows` (<= 0: one chunk) and
+ * sum_out / sumsq_out are [n_chunks x d].  All chunks run in one launch; the
+ * accumulation order is fixed, so results are deterministic.
  * Replaces: assay.py:812-828 (RNAassay.normed with renormalize_subset),
  *           assay.py:41-64 (norm_lib_size[_log]),
  *           graph_datastore.py:767-796 (data.mean(0) / data.std(0), per chunk). */
 int sh_norm_stats(const uint32_t *vals, const int32_t *cols, const int64_t *rowptr,
-                  int64_t n_rows, const int32_t *col_map, int32_t n_cols_in, int32_t d,
-                  double sf, int log_transform, const double *libsize_in,
-                  double *libsize_out, double *sum_out, double *sumsq_out,
-                  void *workspace, size_t workspace_bytes, sh_stream_t stream);
-size_t sh_norm_stats_workspace(int64_t n_rows, int32_t d);
+                  int64_t n_rows, int64_t chunk_rows, const int32_t *col_map,
+                  int32_t n_cols_in, int32_t d, double sf, int log_transform,
+                  const double *libsize_in, double *libsize_out, double *sum_out,
+                  double *sumsq_out, void *workspace, size_t workspace_bytes,
+                  sh_stream_t stream);
+size_t sh_norm_stats_workspace(int64_t n_rows, int64_t chunk_rows, int32_t d);
 
 /* Merges per-chunk (count, sum, sumsq) [n_chunks][d] with Chan's pairwise formula
  * into mu and sigma (population, ddof=0), then applies clean_array: NaN/inf -> 0,
@@ -77,7 +80,8 @@ int sh_stats_merge(const double *sums, const double *sumsqs, const int64_t *coun
 
 /* Dense tile of z = (x - center) * scale (clipped to +-clip when clip > 0), FP64
  * (out_f64 != 0) or FP32, for the rows `rows[0..n_rows)` (NULL: rows 0..n_rows-1 of
- * the CSR); columns d..ld-1 are written as 0.  Feeds sh_gram.
+ * the CSR).  Padding columns: column d holds 1.0 when ld > d (so sh_gram's ones_col
+ * yields the column sums), columns d+1..ld-1 hold 0.  Feeds sh_gram.
  * Replaces: ann.py:191-192 (transform_z) on the blocks fed to
  *           IncrementalPCA.partial_fit (ann.py:229-241). */
 int sh_densify(const uint32_t *vals, const int32_t *cols, const int64_t *rowptr,
@@ -88,16 +92,17 @@ int sh_densify(const uint32_t *vals, const int32_t *cols, const int64_t *rowptr,
 
 /* G = Z^T Z (FP64 [d x d], full symmetric) of a dense Z [n x ld] on MFMA: FP64 Z on
  * v_mfma_f64_16x16x4f64 (z_f64 != 0), or FP32 Z on v_mfma_f32_32x32x2f32 with FP32
- * accumulation per row slab; slabs are summed in FP64 in a fixed order.  Optionally
- * the FP64 column sums of Z (colsum_out [d], may be NULL).  accumulate != 0 adds to
- * G_out / colsum_out instead of overwriting them.  ld must be a multiple of 128 with
- * columns d..ld-1 zero (sh_densify writes them so).
+ * accumulation per row slab; slabs are summed in FP64 in a fixed order.  Column sums
+ * of Z go to colsum_out [d] (may be NULL): with ones_col != 0 (Z[:, d] == 1, as
+ * sh_densify writes) they are read off the Gram's column d at no extra pass.
+ * accumulate != 0 adds to G_out / colsum_out instead of overwriting them.  ld must be
+ * a multiple of 128; columns past d must be as sh_densify writes them.
  * Replaces: the SVD inside IncrementalPCA.partial_fit (_incremental_pca.py:345-363,
  *           called from ann.py:241,250): right singular vectors of the augmented
  *           chunk are eigenvectors of its Gram matrix. */
-int sh_gram(const void *Z, int z_f64, int64_t n, int32_t d, int64_t ld, int accumulate,
-            double *G_out, double *colsum_out, void *workspace, size_t workspace_bytes,
-            sh_stream_t stream);
+int sh_gram(const void *Z, int z_f64, int64_t n, int32_t d, int64_t ld, int ones_col,
+            int accumulate, double *G_out, double *colsum_out, void *workspace,
+            size_t workspace_bytes, sh_stream_t stream);
 size_t sh_gram_workspace(int64_t n, int32_t d, int z_f64);
 
 /* Sparse z-score projection: E[r] = sum_f z_rf W[f] with z = (x - mu) * inv_sigma

@@ -37,11 +37,11 @@ _sz = ctypes.c_size_t
 _SIGS = {
     "sh_version": (_int, []),
     "sh_last_error": (ctypes.c_char_p, []),
-    "sh_norm_stats": (_int, [_p, _p, _p, _i64, _p, _i32, _i32, _f64, _int, _p, _p, _p, _p, _p, _sz, _p]),
-    "sh_norm_stats_workspace": (_sz, [_i64, _i32]),
+    "sh_norm_stats": (_int, [_p, _p, _p, _i64, _i64, _p, _i32, _i32, _f64, _int, _p, _p, _p, _p, _p, _sz, _p]),
+    "sh_norm_stats_workspace": (_sz, [_i64, _i64, _i32]),
     "sh_stats_merge": (_int, [_p, _p, _p, _i32, _i32, _p, _p, _p, _p]),
     "sh_densify": (_int, [_p, _p, _p, _p, _i64, _p, _i32, _i32, _f64, _int, _p, _p, _p, _f64, _int, _p, _i64, _p]),
-    "sh_gram": (_int, [_p, _int, _i64, _i32, _i64, _int, _p, _p, _p, _sz, _p]),
+    "sh_gram": (_int, [_p, _int, _i64, _i32, _i64, _int, _int, _p, _p, _p, _sz, _p]),
     "sh_gram_workspace": (_sz, [_i64, _i32, _int]),
     "sh_project": (_int, [_p, _p, _p, _i64, _p, _i32, _i32, _f64, _int, _p, _p, _p, _f32, _p, _i64, _i32, _p, _p, _i64, _p]),
     "sh_knn_pack_k": (_i32, [_int, _i32]),

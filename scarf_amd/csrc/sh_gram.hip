@@ -220,10 +220,13 @@ __global__ void __launch_bounds__(256) k_gram64(const double *__restrict__ Z, in
       }
 }
 
+// Sums the slab partials of the upper-triangle tiles into G (d x d).  With a ones
+// column at index d (ones_col), entry (i, d) of the extended Gram is the column sum i.
 template <typename PT>
 __global__ void __launch_bounds__(256) k_gram_reduce(const PT *__restrict__ part, int T, int S,
-                                                     int nb, int32_t d, int accumulate,
-                                                     double *__restrict__ G) {
+                                                     int nb, int32_t d, int ones_col,
+                                                     int accumulate, double *__restrict__ G,
+                                                     double *__restrict__ colsum) {
   const int t = blockIdx.y;
   const int e = blockIdx.x * 256 + threadIdx.x;  // element within the tile
   if (e >= kGT * kGT) return;
@@ -231,10 +234,15 @@ __global__ void __launch_bounds__(256) k_gram_reduce(const PT *__restrict__ part
   tile_coords(t, nb, bi, bj);
   const int m = e / kGT, nn = e % kGT;
   const int i = bi * kGT + m, j = bj * kGT + nn;
-  if (i >= d || j >= d) return;
+  const bool col = ones_col && j == d && i < d;
+  if (!col && (i >= d || j >= d)) return;
   if (bi == bj && m > nn) return;  // diagonal tiles: upper half only
   double s = 0.0;
   for (int sl = 0; sl < S; ++sl) s += (double)part[((int64_t)sl * T + t) * (kGT * kGT) + e];
+  if (col) {
+    colsum[i] = accumulate ? colsum[i] + s : s;
+    return;
+  }
   if (accumulate) s += G[(int64_t)i * d + j];
   G[(int64_t)i * d + j] = s;
   G[(int64_t)j * d + i] = s;
@@ -292,19 +300,21 @@ using namespace sh;
 
 extern "C" size_t sh_gram_workspace(int64_t n, int32_t d, int z_f64) {
   if (d <= 0) return 0;
-  GramPlan p = gram_plan(n, d, z_f64);
+  GramPlan p = gram_plan(n, d + 1, z_f64);  // room for the ones column
   return p.part_bytes + p.col_bytes;
 }
 
-extern "C" int sh_gram(const void *Z, int z_f64, int64_t n, int32_t d, int64_t ld, int accumulate,
-                       double *G_out, double *colsum_out, void *workspace,
+extern "C" int sh_gram(const void *Z, int z_f64, int64_t n, int32_t d, int64_t ld, int ones_col,
+                       int accumulate, double *G_out, double *colsum_out, void *workspace,
                        size_t workspace_bytes, sh_stream_t stream) {
   hipStream_t st = (hipStream_t)stream;
   SH_ARG(n >= 0 && d > 0, "sh_gram: bad sizes n=%lld d=%d", (long long)n, d);
   SH_ARG(ld % kGT == 0 && ld >= d, "sh_gram: ld=%lld must be a multiple of 128 and >= d",
          (long long)ld);
   SH_ARG(Z && G_out, "sh_gram: NULL pointer");
-  GramPlan p = gram_plan(n, d, z_f64);
+  SH_ARG(!ones_col || (ld > d && colsum_out), "sh_gram: ones_col needs ld > d and colsum_out");
+  const int32_t de = ones_col ? d + 1 : d;  // Gram extent including the ones column
+  GramPlan p = gram_plan(n, de, z_f64);
   SH_ARG((int64_t)p.nb * kGT <= ld, "sh_gram: ld too small for the tile grid");
   if (!workspace || workspace_bytes < p.part_bytes + p.col_bytes) {
     set_error("sh_gram: workspace %zu < %zu bytes", workspace_bytes, p.part_bytes + p.col_bytes);
@@ -316,17 +326,17 @@ extern "C" int sh_gram(const void *Z, int z_f64, int64_t n, int32_t d, int64_t l
                        p.nb, p.T, p.S, p.rows_per_slab, part);
     SH_LAUNCH_CHECK("k_gram64");
     hipLaunchKernelGGL(k_gram_reduce<double>, dim3(kGT * kGT / 256, p.T), dim3(256), 0, st, part,
-                       p.T, p.S, p.nb, d, accumulate, G_out);
+                       p.T, p.S, p.nb, d, ones_col, accumulate, G_out, colsum_out);
   } else {
     float *part = (float *)workspace;
     hipLaunchKernelGGL(k_gram, dim3(p.T * p.S), dim3(256), 0, st, (const float *)Z, n, ld, p.nb,
                        p.T, p.S, p.rows_per_slab, part);
     SH_LAUNCH_CHECK("k_gram");
     hipLaunchKernelGGL(k_gram_reduce<float>, dim3(kGT * kGT / 256, p.T), dim3(256), 0, st, part,
-                       p.T, p.S, p.nb, d, accumulate, G_out);
+                       p.T, p.S, p.nb, d, ones_col, accumulate, G_out, colsum_out);
   }
   SH_LAUNCH_CHECK("k_gram_reduce");
-  if (colsum_out) {
+  if (colsum_out && !ones_col) {
     double *cp = (double *)((char *)workspace + p.part_bytes);
     if (z_f64)
       hipLaunchKernelGGL(k_colsum_part<double>, dim3(cdiv(d, 256), kColParts), dim3(256), 0, st,

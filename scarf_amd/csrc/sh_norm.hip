@@ -42,14 +42,16 @@ __global__ void __launch_bounds__(256) k_libsize(const uint32_t *__restrict__ va
 }
 
 // Per-feature FP64 sums of x and x^2 over a slab of rows, for columns [c0, c0+cb).
+// Slabs never straddle a chunk: slab s covers rows [c*chunk + j*slab_rows, ...) of
+// chunk c = s / spc, j = s % spc, so every chunk of the matrix is done in one launch.
 // Each wave owns a private LDS accumulator; within a row every mapped column is
 // distinct, so the read-modify-write needs no atomics and the order is fixed.
 constexpr int kStatsWaves = 2;
 __global__ void __launch_bounds__(64 * kStatsWaves) k_norm_stats(
     const uint32_t *__restrict__ vals, const int32_t *__restrict__ cols,
-    const int64_t *__restrict__ rowptr, int64_t n_rows, int64_t rows_per_slab,
-    const int32_t *__restrict__ col_map, int32_t n_in, int32_t d, int32_t cb,
-    double sf, int log_t, const double *__restrict__ libsize,
+    const int64_t *__restrict__ rowptr, int64_t n_rows, int64_t chunk_rows, int spc,
+    int64_t slab_rows, const int32_t *__restrict__ col_map, int32_t n_in, int32_t d,
+    int32_t cb, double sf, int log_t, const double *__restrict__ libsize,
     double *__restrict__ part /* [n_slabs][2][d] */) {
   extern __shared__ double acc[];  // [kStatsWaves][2][cb]
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -59,8 +61,10 @@ __global__ void __launch_bounds__(64 * kStatsWaves) k_norm_stats(
   double *my = acc + (size_t)wave * 2 * cb;
   for (int i = threadIdx.x; i < kStatsWaves * 2 * cb; i += blockDim.x) acc[i] = 0.0;
   __syncthreads();
-  const int64_t r0 = (int64_t)blockIdx.x * rows_per_slab;
-  const int64_t r1 = min(n_rows, r0 + rows_per_slab);
+  const int64_t chunk = blockIdx.x / spc, j_in = blockIdx.x % spc;
+  const int64_t c_end = min(n_rows, (chunk + 1) * chunk_rows);
+  const int64_t r0 = chunk * chunk_rows + j_in * slab_rows;
+  const int64_t r1 = min(c_end, r0 + slab_rows);
   for (int64_t r = r0 + wave; r < r1; r += kStatsWaves) {
     const int64_t a = rowptr[r], b = rowptr[r + 1];
     const double l = libsize[r];
@@ -90,17 +94,22 @@ __global__ void __launch_bounds__(64 * kStatsWaves) k_norm_stats(
   }
 }
 
-__global__ void k_reduce_slabs(const double *__restrict__ part, int n_slabs, int32_t d,
-                               double *__restrict__ sum_out, double *__restrict__ sq_out) {
-  int f = blockIdx.x * blockDim.x + threadIdx.x;
-  if (f >= d) return;
+// One thread per (chunk, feature): the chunk's slabs summed in slab order.
+__global__ void k_reduce_slabs(const double *__restrict__ part, int spc, int64_t n_chunks,
+                               int32_t d, double *__restrict__ sum_out,
+                               double *__restrict__ sq_out) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_chunks * d) return;
+  const int64_t c = t / d;
+  const int f = (int)(t % d);
   double s = 0.0, q = 0.0;
-  for (int i = 0; i < n_slabs; ++i) {
-    s += part[(size_t)i * 2 * d + f];
-    q += part[(size_t)i * 2 * d + d + f];
+  for (int i = 0; i < spc; ++i) {
+    const size_t base = (size_t)(c * spc + i) * 2 * d;
+    s += part[base + f];
+    q += part[base + d + f];
   }
-  sum_out[f] = s;
-  sq_out[f] = q;
+  sum_out[t] = s;
+  sq_out[t] = q;
 }
 
 // clean_array (utils.py:143-154): nan_to_num maps NaN -> 0 and +-inf -> +-DBL_MAX
@@ -158,7 +167,7 @@ __global__ void __launch_bounds__(256) k_densify(
     if (i >= n_rows) break;
     const int64_t r = rows ? rows[i] : i;
     for (int64_t f = threadIdx.x; f < ld; f += blockDim.x) {
-      double v = 0.0;
+      double v = f == d ? 1.0 : 0.0;  // column d (padding) holds ones: Z^T Z gets column sums
       if (f < d) {
         v = (0.0 - center[f]) * scale[f];
         if (clip > 0.0) v = fmin(fmax(v, -clip), clip);
@@ -257,20 +266,30 @@ __global__ void __launch_bounds__(256) k_project(
 
 using namespace sh;
 
-static int64_t stats_slabs(int64_t n_rows) {
-  int64_t s = n_rows / 64;
-  if (s < 1) s = 1;
-  if (s > 1024) s = 1024;
-  return s;
+struct StatsPlan {
+  int64_t n_chunks, spc, slab_rows;
+};
+static StatsPlan stats_plan(int64_t n_rows, int64_t chunk_rows) {
+  StatsPlan p;
+  if (chunk_rows <= 0 || chunk_rows > n_rows) chunk_rows = n_rows > 0 ? n_rows : 1;
+  p.n_chunks = cdiv(n_rows > 0 ? n_rows : 1, chunk_rows);
+  // ~256 rows per slab, at most 64 slabs per chunk
+  int64_t spc = cdiv(chunk_rows, 256);
+  if (spc < 1) spc = 1;
+  if (spc > 64) spc = 64;
+  p.spc = spc;
+  p.slab_rows = cdiv(chunk_rows, spc);
+  return p;
 }
 
-extern "C" size_t sh_norm_stats_workspace(int64_t n_rows, int32_t d) {
-  return (size_t)stats_slabs(n_rows) * 2 * (size_t)(d > 0 ? d : 1) * sizeof(double);
+extern "C" size_t sh_norm_stats_workspace(int64_t n_rows, int64_t chunk_rows, int32_t d) {
+  StatsPlan p = stats_plan(n_rows, chunk_rows);
+  return (size_t)(p.n_chunks * p.spc) * 2 * (size_t)(d > 0 ? d : 1) * sizeof(double);
 }
 
 extern "C" int sh_norm_stats(const uint32_t *vals, const int32_t *cols, const int64_t *rowptr,
-                             int64_t n_rows, const int32_t *col_map, int32_t n_cols_in,
-                             int32_t d, double sf, int log_transform,
+                             int64_t n_rows, int64_t chunk_rows, const int32_t *col_map,
+                             int32_t n_cols_in, int32_t d, double sf, int log_transform,
                              const double *libsize_in, double *libsize_out, double *sum_out,
                              double *sumsq_out, void *workspace, size_t workspace_bytes,
                              sh_stream_t stream) {
@@ -285,7 +304,9 @@ extern "C" int sh_norm_stats(const uint32_t *vals, const int32_t *cols, const in
     SH_HIP(hipMemsetAsync(sumsq_out, 0, sizeof(double) * d, st));
     return SH_OK;
   }
-  const size_t need = sh_norm_stats_workspace(n_rows, d);
+  const StatsPlan plan = stats_plan(n_rows, chunk_rows);
+  const int64_t chunk = (chunk_rows > 0 && chunk_rows < n_rows) ? chunk_rows : n_rows;
+  const size_t need = sh_norm_stats_workspace(n_rows, chunk_rows, d);
   if (workspace_bytes < need || !workspace) {
     set_error("sh_norm_stats: workspace %zu < %zu bytes", workspace_bytes, need);
     return SH_EWORKSPACE;
@@ -301,18 +322,17 @@ extern "C" int sh_norm_stats(const uint32_t *vals, const int32_t *cols, const in
     SH_HIP(hipMemcpyAsync(libsize_out, libsize_in, sizeof(double) * n_rows,
                           hipMemcpyDeviceToDevice, st));
   }
-  const int64_t slabs = stats_slabs(n_rows);
-  const int64_t rps = cdiv(n_rows, slabs);
   const int32_t cb = d < 2048 ? d : 2048;  // 2 waves x 2 x 2048 doubles = 64 KiB
   const int n_cb = (int)cdiv(d, cb);
   const size_t lds = (size_t)kStatsWaves * 2 * cb * sizeof(double);
   double *part = (double *)workspace;
-  hipLaunchKernelGGL(k_norm_stats, dim3((unsigned)slabs, n_cb), dim3(64 * kStatsWaves), lds, st,
-                     vals, cols, rowptr, n_rows, rps, col_map, n_cols_in, d, cb, sf,
-                     log_transform, ls, part);
+  hipLaunchKernelGGL(k_norm_stats, dim3((unsigned)(plan.n_chunks * plan.spc), n_cb),
+                     dim3(64 * kStatsWaves), lds, st, vals, cols, rowptr, n_rows, chunk,
+                     (int)plan.spc, plan.slab_rows, col_map, n_cols_in, d, cb, sf, log_transform,
+                     ls, part);
   SH_LAUNCH_CHECK("k_norm_stats");
-  hipLaunchKernelGGL(k_reduce_slabs, dim3(cdiv(d, 256)), dim3(256), 0, st, part, (int)slabs, d,
-                     sum_out, sumsq_out);
+  hipLaunchKernelGGL(k_reduce_slabs, dim3(cdiv(plan.n_chunks * d, 256)), dim3(256), 0, st, part,
+                     (int)plan.spc, plan.n_chunks, d, sum_out, sumsq_out);
   SH_LAUNCH_CHECK("k_reduce_slabs");
   return SH_OK;
 }

@@ -76,14 +76,12 @@ def feature_stats(csr: DeviceCSR, chunk_rows: int, params: NormParams, col_map=N
             raise ValueError("renormalize_subset=False needs the per-cell total counts")
         libsize[:n] = n_counts.to(device=dev, dtype=torch.float64)
     st = stream_handle(stream)
-    ws = workspace("stats")
-    for c, (s, e) in enumerate(bounds):
-        buf, nbytes = ws.get(_lib.lib().sh_norm_stats_workspace(e - s, d))
-        _lib.call("sh_norm_stats", ptr(csr.vals), ptr(csr.cols), ptr(csr.rowptr, s), e - s,
-                  ptr(col_map), csr.n_cols, d, float(params.size_factor),
-                  int(params.log_transform),
-                  ptr(libsize, s) if not params.renormalize_subset else None,
-                  ptr(libsize, s), ptr(sums[c]), ptr(sqs[c]), ptr(buf), nbytes, st)
+    buf, nbytes = workspace("stats").get(
+        _lib.lib().sh_norm_stats_workspace(n, int(chunk_rows), d))
+    _lib.call("sh_norm_stats", ptr(csr.vals), ptr(csr.cols), ptr(csr.rowptr), n, int(chunk_rows),
+              ptr(col_map), csr.n_cols, d, float(params.size_factor), int(params.log_transform),
+              ptr(libsize) if not params.renormalize_subset else None, ptr(libsize), ptr(sums),
+              ptr(sqs), ptr(buf), nbytes, st)
     mu = torch.empty(d, dtype=torch.float64, device=dev)
     sigma = torch.empty(d, dtype=torch.float64, device=dev)
     cmean = torch.empty((nc, d), dtype=torch.float64, device=dev)

