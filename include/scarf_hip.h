@@ -105,6 +105,41 @@ int sh_gram(const void *Z, int z_f64, int64_t n, int32_t d, int64_t ld, int ones
             size_t workspace_bytes, sh_stream_t stream);
 size_t sh_gram_workspace(int64_t n, int32_t d, int z_f64);
 
+/* ---- symmetric eigensolve of the PCA merge matrix (FP64) --------------------------
+ * The top (dims + 1) eigenpairs of the d x d merge matrix replace the SVD of the
+ * augmented chunk in IncrementalPCA.partial_fit (_incremental_pca.py:345-363, called
+ * from ann.py:241,250).  The d-sized work runs here; the caller solves the small
+ * block-tridiagonal problem and tests convergence.
+ *
+ * sh_merge_matrix: M = (G - n bm bm^T) + Vt^T diag(s2) Vt + mc mc^T, all FP64 d x d
+ * row-major (G symmetric); Vt [nc x d] may be NULL (first fit: no previous
+ * components), mc may be NULL.  Restates _incremental_pca.py:335-356 (the chunk
+ * centring, the previous components scaled by their singular values and the
+ * mean-correction row), formed as a Gram instead of stacked rows. */
+int sh_merge_matrix(const double *G, int32_t d, double n, const double *bm,
+                    const double *Vt, const double *s2, int32_t nc, const double *mc,
+                    double *M, sh_stream_t stream);
+
+/* Block Lanczos (block size 8) with full two-pass classical Gram-Schmidt and
+ * CholeskyQR2: runs block steps j0 .. j0 + n_steps - 1 on M (d x d symmetric).
+ * V [kmax x d] holds the orthonormal basis, one vector per row; rows 0..7 are the
+ * caller's starting block.  Step j writes rows 8(j+1)..8(j+1)+7, the diagonal block
+ * T_diag[j] = V_j M V_j^T (8 x 8) and T_sub[j] = R_j (8 x 8 upper triangular, the block
+ * below the diagonal of the Lanczos matrix).  status (i32, set to 0 by the caller)
+ * becomes 1 when a new block is numerically rank deficient (basis exhausted).
+ * Needs 8 (j0 + n_steps + 1) <= kmax and kmax % 8 == 0. */
+size_t sh_lanczos_workspace(int32_t d, int32_t kmax);
+int sh_lanczos_steps(const double *M, int32_t d, double *V, int32_t kmax, int32_t j0,
+                     int32_t n_steps, double *T_diag, double *T_sub, int32_t *status,
+                     void *workspace, size_t workspace_bytes, sh_stream_t stream);
+
+/* FP64 GEMM on MFMA: C[m x n] = alpha op(A) B + beta C, op(A) = A (m x k, transA 0)
+ * or A^T (A stored k x m, transA 1), row-major.  Used for the Ritz vectors
+ * (S^T V) and their residual check (Vt M). */
+int sh_dgemm(int transA, int64_t m, int64_t n, int64_t k, double alpha, const double *A,
+             int64_t lda, const double *B, int64_t ldb, double beta, double *C, int64_t ldc,
+             sh_stream_t stream);
+
 /* Sparse z-score projection: E[r] = sum_f z_rf W[f] with z = (x - mu) * inv_sigma
  * (clipped to +-clip when clip > 0), evaluated as c + sum over the row's non-zeros of
  * (z(x) - z(0)) * W[f], where c = sum_f z(0)_f W[f] (+ alignment fill, see mapping)

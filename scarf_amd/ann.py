@@ -101,10 +101,11 @@ class NormedCSR:
         c = torch.zeros(d, dtype=torch.float64, device=dev) if center is None else center
         sc = torch.ones(d, dtype=torch.float64, device=dev) if scale is None else scale
         rows = torch.arange(self.row_start, self.row_end, device=dev, dtype=torch.int64)
+        c, sc = c.contiguous(), sc.contiguous()  # referenced until the launch is queued
         _lib.call("sh_densify", ptr(self.counts.vals), ptr(self.counts.cols),
                   ptr(self.counts.rowptr), ptr(rows), n, ptr(self._cm), self.counts.n_cols, d,
                   float(self.norm.size_factor), int(self.norm.log_transform), ptr(st.libsize),
-                  ptr(c.contiguous()), ptr(sc.contiguous()),
+                  ptr(c), ptr(sc),
                   float(self.norm.clip) if self.norm.clip else 0.0, 1, ptr(out), ld,
                   stream_handle())
         return out[:n, :d].cpu().numpy()

@@ -50,6 +50,7 @@ class GraphParams:
     bandwidth: float = 1.5
     pca_mode: str = "ipca"  # "ipca" (scarf's block order) or "exact"
     gram_precision: str = "fp64"
+    pca_eig: str = "auto"  # "auto" | "device" (GPU block Lanczos) | "host" (LAPACK)
     screen: str = "bf16"  # "fp32" = certified exact, "bf16" = screened
     pool: int = 64
 
@@ -126,7 +127,7 @@ def build_graph(csr: DeviceCSR, params: GraphParams, feat_idx=None,
         fit = fit_pca(csr, stats, p, dims, params.batch_size, use_for_pca=use_for_pca,
                       mode=params.pca_mode, col_map=cm,
                       disable_scaling=not params.feat_scaling,
-                      precision=params.gram_precision, stream=stream)
+                      precision=params.gram_precision, eig=params.pca_eig, stream=stream)
     else:
         fit = PcaFit(np.asarray(loadings, dtype=np.float64), None, "given")
     timer.mark("pca_fit")

@@ -194,8 +194,10 @@ def fix_knn_query_device(idx, dist, ref_idx, stream=None):
     oi = torch.empty((n, k1 - 1), dtype=torch.int64, device=dev)
     od = torch.empty((n, k1 - 1), dtype=torch.float32, device=dev)
     fl = torch.zeros(max(n, 1), dtype=torch.int32, device=dev)
-    _lib.call("sh_fix_knn_query", ptr(idx.contiguous()), ptr(dist.contiguous()),
-              ptr(ref_idx.contiguous()), n, k1, ptr(oi), ptr(od), ptr(fl), stream_handle(stream))
+    # contiguous copies stay referenced until the launch is queued
+    idx, dist, ref_idx = idx.contiguous(), dist.contiguous(), ref_idx.contiguous()
+    _lib.call("sh_fix_knn_query", ptr(idx), ptr(dist), ptr(ref_idx), n, k1, ptr(oi), ptr(od),
+              ptr(fl), stream_handle(stream))
     n_mis = int(((fl[:n] & _lib.SH_FLAG_SELF_NOT_FIRST) > 0).sum().item())
     return oi, od, n_mis
 

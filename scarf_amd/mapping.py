@@ -52,7 +52,8 @@ def mapping_scores(idx: "torch.Tensor", dist: "torch.Tensor", n_ref: int, groups
     ng = len(labels)
     out = torch.empty((ng, n_ref), dtype=torch.float64, device=dev)
     buf, nbytes = workspace("mapping").get(_lib.lib().sh_mapping_score_workspace(n_ref, ng))
-    _lib.call("sh_mapping_score", ptr(idx.contiguous()), ptr(dist.contiguous()), nq, k,
+    idx, dist = idx.contiguous(), dist.contiguous()  # referenced until the launch is queued
+    _lib.call("sh_mapping_score", ptr(idx), ptr(dist), nq, k,
               ptr(g_t), ng, n_ref, int(weighted), float(fixed_weight), float(multiplier),
               int(log_transform), ptr(out), ptr(buf), nbytes, stream_handle(stream))
     o = out.cpu().numpy()

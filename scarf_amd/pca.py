@@ -26,6 +26,7 @@ import numpy as np
 
 from . import _lib
 from .device import DeviceCSR, ptr, require_gpu, stream_handle, workspace
+from .eigen import EigenNotConverged, top_eigh_device
 from .normalize import FeatureStats, NormParams, chunk_bounds
 
 try:
@@ -37,6 +38,11 @@ try:
     import scipy.linalg as _sla
 except ImportError:  # pragma: no cover
     _sla = None
+
+
+# below this many features the host LAPACK solve takes a few milliseconds and the
+# Krylov basis would span most of the space
+EIG_DEVICE_MIN_D = 512
 
 
 def _top_eigh(M: np.ndarray, n_top: int):
@@ -73,25 +79,63 @@ class IPCAState:
         self.explained_variance_: Optional[np.ndarray] = None
         self.explained_variance_ratio_: Optional[np.ndarray] = None
         self.noise_variance_ = 0.0
+        self.eig_log: list = []  # (backend, basis size or d) per partial fit
 
-    def merge(self, n: int, G: np.ndarray, colsum: np.ndarray):
-        """One partial_fit from the uncentred Gram and column sums of the chunk's z."""
-        d = G.shape[0]
+    def merge(self, n: int, G, colsum: np.ndarray, eig: str = "auto", stream=None):
+        """One partial_fit from the uncentred Gram and column sums of the chunk's z.
+
+        ``G`` is a host FP64 array or a device FP64 tensor (then the merge matrix is
+        formed on the GPU).  ``eig``: "device" (GPU block Lanczos), "host" (LAPACK) or
+        "auto" (device for a device ``G`` with d >= EIG_DEVICE_MIN_D, LAPACK when the
+        device solve does not converge within its basis capacity).
+        """
+        on_dev = torch is not None and isinstance(G, torch.Tensor)
+        d = int(G.shape[0])
         if self.components_ is None and self.n_components > n:
             raise ValueError(f"n_components={self.n_components} must be <= the first batch's "
                              f"sample count {n}")
         if self.n_components > d:
             raise ValueError(f"n_components={self.n_components} invalid for n_features={d}")
+        if eig not in ("auto", "device", "host"):
+            raise ValueError(f"unknown eigensolver {eig!r}")
+        colsum = np.asarray(colsum, dtype=np.float64)
         bm = colsum / n
-        C = G - n * np.outer(bm, bm)
-        C = 0.5 * (C + C.T)
-        new_unnorm = np.clip(np.diag(C).copy(), 0.0, None)
         n_seen = float(self.n_samples_seen_)
         n_total = n_seen + n
         if n_seen == 0:
+            mc = None
+        else:
+            mc = math.sqrt((n_seen / n_total) * n) * (np.asarray(self.mean_) - bm)
+        if on_dev:
+            diag_g = G.diagonal().cpu().numpy()
+            dev = G.device
+            t = lambda a: torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(dev)
+            M = torch.empty_like(G)
+            prev = self.components_ is not None and n_seen > 0
+            # device copies stay referenced until the launch is queued (a temporary's block
+            # could otherwise be handed to the next copy before the kernel reads it)
+            bm_d = t(bm)
+            Vt_d = t(self.components_) if prev else None
+            s2_d = t(self.singular_values_ ** 2) if prev else None
+            mc_d = t(mc) if mc is not None else None
+            _lib.call("sh_merge_matrix", ptr(G), d, float(n), ptr(bm_d), ptr(Vt_d), ptr(s2_d),
+                      self.n_components if prev else 0, ptr(mc_d), ptr(M), stream_handle(stream))
+            trace_m = float(M.diagonal().sum().item())
+        else:
+            diag_g = np.diag(G)
+            C = G - n * np.outer(bm, bm)
+            C = 0.5 * (C + C.T)
+            if n_seen == 0:
+                M = C
+            else:
+                M = (self.components_.T * self.singular_values_ ** 2) @ self.components_
+                M += C
+                M += np.outer(mc, mc)
+            trace_m = float(np.trace(M))
+        new_unnorm = np.clip(diag_g - n * (bm * bm), 0.0, None)
+        if n_seen == 0:
             col_mean = bm
             col_var = new_unnorm / n_total
-            M = C
         else:
             last_sum = np.asarray(self.mean_) * n_seen
             col_mean = (last_sum + colsum) / n_total
@@ -99,12 +143,8 @@ class IPCAState:
             upd = np.asarray(self.var_) * n_seen + new_unnorm + \
                 ratio / n_total * (last_sum / ratio - colsum) ** 2
             col_var = upd / n_total
-            mc = math.sqrt((n_seen / n_total) * n) * (np.asarray(self.mean_) - bm)
-            M = (self.components_.T * self.singular_values_ ** 2) @ self.components_
-            M += C
-            M += np.outer(mc, mc)
         nc = self.n_components
-        w, Vt = _top_eigh(M, nc)  # raises LinAlgError on failure, like the SVD
+        w, Vt = self._eigensolve(M, nc, on_dev, eig, stream)
         w = np.clip(w, 0.0, None)
         Vt = _flip_rows(Vt)
         S = np.sqrt(w)
@@ -119,10 +159,27 @@ class IPCAState:
         self.explained_variance_ratio_ = S ** 2 / tot if tot > 0 else np.zeros_like(S)
         rank = min(rows_aug, d)
         if nc not in (rows_aug, d) and rank > nc:
-            rest = max(float(np.trace(M)) - float(w.sum()), 0.0)
+            rest = max(trace_m - float(w.sum()), 0.0)
             self.noise_variance_ = rest / (n_total - 1) / (rank - nc)
         else:
             self.noise_variance_ = 0.0
+
+    def _eigensolve(self, M, nc: int, on_dev: bool, eig: str, stream):
+        d = int(M.shape[0])
+        use_dev = on_dev and (eig == "device" or (eig == "auto" and d >= EIG_DEVICE_MIN_D))
+        if eig == "device" and not on_dev:
+            raise ValueError("eig='device' needs the Gram on the GPU")
+        if use_dev:
+            try:
+                w, Vt, info = top_eigh_device(M, nc, stream=stream)
+                self.eig_log.append(("device", info.vectors))
+                return w, Vt
+            except EigenNotConverged:
+                if eig == "device":
+                    raise
+        Mh = M.cpu().numpy() if on_dev else M
+        self.eig_log.append(("host", d))
+        return _top_eigh(Mh, nc)  # raises LinAlgError on failure, like the SVD
 
 
 def scarf_partial_fit_calls(block_rows: List[np.ndarray], dims: int):
@@ -164,8 +221,9 @@ def _z_params(stats: FeatureStats, disable_scaling: bool):
 
 def gram_of_rows(csr: DeviceCSR, stats: FeatureStats, params: NormParams, rows,
                  col_map=None, disable_scaling=False, sub_rows: int = 32768,
-                 precision: str = "fp64", stream=None):
-    """(Z^T Z, column sums, n) of z over ``rows`` (device int64 tensor), FP64 host arrays.
+                 precision: str = "fp64", stream=None, to_host: bool = True):
+    """(Z^T Z, column sums, n) of z over ``rows`` (device int64 tensor) in FP64: host
+    arrays, or with ``to_host=False`` the Gram as a device tensor (column sums on host).
 
     ``precision="fp64"`` densifies z in FP64 and uses FP64 MFMA (parity with the
     reference's FP64 PCA on every component); ``"fp32"`` rounds z to FP32 first.
@@ -195,14 +253,17 @@ def gram_of_rows(csr: DeviceCSR, stats: FeatureStats, params: NormParams, rows,
         buf, nbytes = ws.get(_lib.lib().sh_gram_workspace(b - a, d, int(f64)))
         _lib.call("sh_gram", ptr(Z), int(f64), b - a, d, ld, 1, int(a > 0), ptr(G), ptr(cs),
                   ptr(buf), nbytes, st)
-    return G.cpu().numpy(), cs.cpu().numpy(), n
+    return (G.cpu().numpy() if to_host else G), cs.cpu().numpy(), n
 
 
 def fit_pca(csr: DeviceCSR, stats: FeatureStats, params: NormParams, dims: int,
             chunk_rows: int, use_for_pca: Optional[np.ndarray] = None, mode: str = "ipca",
             col_map=None, disable_scaling: bool = False, precision: str = "fp64",
-            stream=None) -> PcaFit:
-    """Fits dims+1 components and keeps the first ``dims`` as loadings (ann.py:212,256)."""
+            eig: str = "auto", stream=None) -> PcaFit:
+    """Fits dims+1 components and keeps the first ``dims`` as loadings (ann.py:212,256).
+
+    ``eig`` selects the eigensolver of each partial fit (see ``IPCAState.merge``).
+    """
     require_gpu()
     n = csr.n_rows
     dev = csr.rowptr.device
@@ -221,8 +282,8 @@ def fit_pca(csr: DeviceCSR, stats: FeatureStats, params: NormParams, dims: int,
     def fit_call(rows_np):
         rows = torch.from_numpy(np.ascontiguousarray(rows_np)).to(dev)
         G, cs, m = gram_of_rows(csr, stats, params, rows, col_map, disable_scaling,
-                                precision=precision, stream=stream)
-        state.merge(m, G, cs)
+                                precision=precision, stream=stream, to_host=eig == "host")
+        state.merge(m, G, cs, eig=eig, stream=stream)
 
     if mode == "exact":
         rows_all = np.concatenate(blocks) if blocks else np.zeros(0, np.int64)

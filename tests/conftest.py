@@ -12,6 +12,11 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (ROCm GPU) and libscarfhip.so")
 
 
+def say(*a):
+    """Names the case before a GPU step (flushed, so a failing run shows where it was)."""
+    print(*a, flush=True)
+
+
 def gpu_available():
     try:
         import torch

@@ -7,7 +7,7 @@ the two-pass mean/std of the normalised data, the first block held back and fitt
 last, and dims + 1 components with the last one dropped.  The normalisation follows
 assay.py:54-64 / 812-828.  Inputs are seeded negative-binomial counts.
 
-Run from the repository root:  python tests/golden/make_ipca_golden.py
+Run from the repository root:  python tests/golden/make_ipca_golden.py [case ...]
 """
 import os
 import sys
@@ -25,6 +25,8 @@ CASES = {
     # name: (cells, genes, groups, density, chunk, dims, sf, seed)
     "ipca_small": (600, 40, 3, 0.2, 150, 10, 1000.0, 11),
     "ipca_blocks": (1000, 120, 4, 0.1, 230, 20, 10000.0, 12),
+    # wide enough (600 genes) for the GPU eigensolver's default size range
+    "ipca_wide": (2000, 600, 6, 0.08, 500, 30, 10000.0, 13),
 }
 
 
@@ -74,5 +76,5 @@ def make(name):
 
 
 if __name__ == "__main__":
-    for c in CASES:
+    for c in (sys.argv[1:] or CASES):
         make(c)

@@ -91,7 +91,7 @@ def test_gram_parity(cuda, data, precision):
     np.testing.assert_allclose(cs, z.sum(0), atol=1e-9 if precision == "fp64" else 2e-3)
 
 
-@pytest.mark.parametrize("name", ["ipca_small", "ipca_blocks"])
+@pytest.mark.parametrize("name", ["ipca_small", "ipca_blocks", "ipca_wide"])
 def test_ipca_golden_parity(cuda, name):
     """Loadings and PCA scores vs scikit-learn IncrementalPCA in scarf's order."""
     from scarf_amd.device import DeviceCSR

@@ -102,7 +102,7 @@ def test_scarf_block_order():
     assert calls == [[2], [0, 1, 3]]
 
 
-@pytest.mark.parametrize("name", ["ipca_small", "ipca_blocks"])
+@pytest.mark.parametrize("name", ["ipca_small", "ipca_blocks", "ipca_wide"])
 def test_ipca_golden(name):
     g = np.load(os.path.join(GOLD, name + ".npz"))
     from scarf_amd.synth import CSR
