@@ -85,3 +85,70 @@ def test_band_storage(u):
     ab = eigen._band_of(T, u)
     np.testing.assert_allclose(sla.eig_banded(ab, lower=False, eigvals_only=True),
                                np.linalg.eigvalsh(T), rtol=1e-12, atol=1e-12)
+
+
+class _NumpyLanczos:
+    """numpy stand-in for the GPU backend of thick_restart_lanczos (same conventions)."""
+
+    def __init__(self, M, kin, seed=0):
+        self.M, self.d = M, M.shape[0]
+        rng = np.random.default_rng(seed)
+        self.V = np.zeros((kin, self.d))
+        self.V[:P] = np.linalg.qr(rng.standard_normal((self.d, P)))[0].T
+        self.T_diag = np.zeros((kin // P, P, P))
+        self.T_sub = np.zeros((kin // P, P, P))
+
+    def steps(self, j0, j1):
+        for j in range(j0, j1):
+            k = (j + 1) * P
+            W = self.V[j * P:k] @ self.M
+            B = self.V[:k]
+            H = B @ W.T
+            W = W - H.T @ B
+            H2 = B @ W.T
+            W = W - H2.T @ B
+            self.T_diag[j] = (H + H2)[j * P:k]
+            Q, R = np.linalg.qr(W.T)
+            sg = np.sign(np.diag(R))
+            self.V[k:k + P] = (Q * sg).T
+            self.T_sub[j] = R * sg[:, None]
+
+    def read(self, j0, j1):
+        return self.T_diag[j0:j1].copy(), self.T_sub[j0:j1].copy(), False
+
+    def restart(self, S, k, m):
+        Y = S.T @ self.V[:k]
+        self.V[m:m + P] = self.V[k:k + P]
+        self.V[:m] = Y
+
+    def ritz(self, S, k):
+        X = S.T @ self.V[:k]
+        Y = X @ self.M
+        th = np.einsum("ij,ij->i", Y, X)
+        o = np.argsort(-th, kind="stable")
+        th, X, Y = th[o], X[o], Y[o]
+        res = np.linalg.norm(Y - th[:, None] * X, axis=1).max() / th[0]
+        orth = np.abs(X @ X.T - np.eye(X.shape[0])).max()
+        return th, X, res, orth
+
+
+@pytest.mark.parametrize("d,nt", [(400, 10), (600, 31), (300, 1)])
+def test_thick_restart_lanczos_host_loop(d, nt):
+    """The restart loop (projected matrix diag(theta) + coupling b + new Lanczos blocks,
+    residual bound ||R s_last||) converges to LAPACK's top eigenpairs."""
+    M = _spectrum_matrix(d, seed=5)
+    keep, kin = eigen.restart_sizes(nt, d)
+    w, Vt, info = eigen.thick_restart_lanczos(_NumpyLanczos(M, kin), nt, keep, kin, 1e-12)
+    w_ref, V_ref = sla.eigh(M, subset_by_index=[d - nt, d - 1])
+    w_ref, V_ref = w_ref[::-1], V_ref[:, ::-1].T
+    np.testing.assert_allclose(w, w_ref, rtol=1e-12)
+    s = np.sign(np.sum(Vt * V_ref, axis=1))
+    np.testing.assert_allclose(s[:, None] * Vt, V_ref, atol=1e-8)
+    assert info.max_residual <= 1e-10 and info.cycles >= 1
+
+
+def test_restart_sizes():
+    keep, kin = eigen.restart_sizes(51, 2000)
+    assert keep % P == 0 and kin % P == 0 and keep >= 51 + 16 and kin >= 2 * keep
+    with pytest.raises(eigen.EigenNotConverged):
+        eigen.restart_sizes(51, 100)

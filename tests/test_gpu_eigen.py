@@ -55,7 +55,7 @@ def test_top_eigh_device_vs_lapack(cuda, case):
     w_ref, V_ref = w_ref[::-1], V_ref[:, ::-1].T
     Md = torch.from_numpy(M).to("cuda")
     w, Vt, info = top_eigh_device(Md, nt)
-    say("  vectors", info.vectors, "checks", info.checks, "res", info.max_residual,
+    say("  vectors", info.vectors, "cycles", info.cycles, "res", info.max_residual,
         "orth", info.orthogonality)
     np.testing.assert_allclose(w, w_ref, rtol=1e-12, atol=1e-13 * w_ref[0])
     s = np.sign(np.sum(Vt * V_ref, axis=1))
