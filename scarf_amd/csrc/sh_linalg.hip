@@ -6,9 +6,9 @@
 //
 // Storage: M is d x d row-major and symmetric; the Krylov basis V is [kmax][d] (each
 // basis vector contiguous), block j = rows [8j, 8j + 8).  Per block step j:
-//   W  = M V_j                       (k_symm_part + k_sum_parts: split-K, fixed order)
-//   H  = V_{0..j}^T W, W -= V H      (twice: k_cgs_proj, k_cgs_update_part, k_sum_parts;
-//                                      each pass subtracts its own coefficients)
+//   W  = M V_j                       (k_symm_rows: one wave per 4 rows of M)
+//   H  = V_{0..j}^T W, W -= V H      (twice: k_cgs_proj, k_cgs_update; each pass
+//                                      subtracts its own coefficients)
 //   A_j = sum over both passes of H's block-j rows (diagonal block of the Lanczos matrix)
 //   V_{j+1} R_j = W                  (k_cholqr2, one workgroup; R_j below the diagonal)
 // All reductions have a fixed order, so a run is bit-reproducible.
@@ -20,8 +20,6 @@ namespace sh {
 
 typedef double doublex4 __attribute__((ext_vector_type(4)));
 constexpr int kLP = 8;         // Lanczos block size
-constexpr int kSymSeg = 512;   // max rows of M per split-K segment (LDS: 8 x 512 doubles)
-constexpr int kCgsRows = 16;   // basis vectors per workgroup in k_cgs_proj (4 per wave)
 
 // ---- FP64 GEMM on MFMA ------------------------------------------------------------
 // C[m x n] = alpha * op(A) * B + beta * C, op(A) = A [m x k] or A^T (A stored k x m), all
@@ -122,140 +120,129 @@ __global__ void k_merge_matrix(const double *__restrict__ G, int32_t d, double n
 }
 
 // ---- block Lanczos pieces -------------------------------------------------------------
-// part[s][q][i] = sum_{j in segment s} M[j][i] V[q][j]   (M symmetric: columns coalesced)
-__global__ void __launch_bounds__(256) k_symm_part(const double *__restrict__ M, int32_t d,
-                                                   const double *__restrict__ V, int32_t seg,
-                                                   double *__restrict__ part) {
-  __shared__ double vs[kLP][kSymSeg];
-  const int s = blockIdx.y;
-  const int j0 = s * seg, j1 = min(d, j0 + seg);
-  for (int e = threadIdx.x; e < kLP * seg; e += 256) {
-    const int q = e / seg, jj = e % seg;
-    vs[q][jj] = (j0 + jj < d) ? V[(int64_t)q * d + j0 + jj] : 0.0;
+// W[q][i] = sum_j M[i][j] V[q][j]: one wave per kRowsPerWave rows of M (contiguous, read
+// once), lanes stride over j; the V values a lane loads serve all of the wave's rows.
+constexpr int kRowsPerWave = 4;
+__global__ void __launch_bounds__(256) k_symm_rows(const double *__restrict__ M, int32_t d,
+                                                   const double *__restrict__ V,
+                                                   double *__restrict__ W) {
+  const int lane = threadIdx.x & 63;
+  const int64_t i0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * kRowsPerWave;
+  if (i0 >= d) return;
+  double acc[kRowsPerWave][kLP];
+#pragma unroll
+  for (int r = 0; r < kRowsPerWave; ++r)
+#pragma unroll
+    for (int q = 0; q < kLP; ++q) acc[r][q] = 0.0;
+  for (int j = lane; j < d; j += 64) {
+    double v[kLP];
+#pragma unroll
+    for (int q = 0; q < kLP; ++q) v[q] = V[(int64_t)q * d + j];
+#pragma unroll
+    for (int r = 0; r < kRowsPerWave; ++r) {
+      const double mv = (i0 + r < d) ? M[(i0 + r) * d + j] : 0.0;
+#pragma unroll
+      for (int q = 0; q < kLP; ++q) acc[r][q] += mv * v[q];
+    }
   }
-  __syncthreads();
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= d) return;
-  double acc[kLP];
 #pragma unroll
-  for (int q = 0; q < kLP; ++q) acc[q] = 0.0;
-  for (int j = j0; j < j1; ++j) {
-    const double mv = M[(int64_t)j * d + i];
+  for (int r = 0; r < kRowsPerWave; ++r)
 #pragma unroll
-    for (int q = 0; q < kLP; ++q) acc[q] += mv * vs[q][j - j0];
-  }
-#pragma unroll
-  for (int q = 0; q < kLP; ++q) part[((int64_t)s * kLP + q) * d + i] = acc[q];
+    for (int q = 0; q < kLP; ++q) {
+      const double s = wave_sum_f64(acc[r][q]);
+      if (lane == 0 && i0 + r < d) W[(int64_t)q * d + i0 + r] = s;
+    }
 }
 
-// out[e] = sign * sum_s part[s][e] (+ out[e] when accumulate), s in order
-__global__ void k_sum_parts(const double *__restrict__ part, int32_t S, int64_t n, double sign,
-                            int accumulate, double *__restrict__ out) {
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= n) return;
-  double s = 0.0;
-  for (int p = 0; p < S; ++p) s += part[(int64_t)p * n + e];
-  out[e] = accumulate ? out[e] + sign * s : sign * s;
-}
-
-// H[c][q] = V[c] . W[q] for c < k (this pass's coefficients; 4 basis vectors per wave).
+// H[c][q] = V[c] . W[q] for c < k (this pass's coefficients; one wave per basis vector).
 // Rows c of block jb (the newest) are also added into T_blk (the diagonal block of the
 // Lanczos matrix, summed over both passes; accumulate = 0 on the first pass).
 __global__ void __launch_bounds__(256) k_cgs_proj(const double *__restrict__ V, int32_t k,
                                                   int32_t d, const double *__restrict__ W,
                                                   double *__restrict__ H, int32_t jb,
                                                   int accumulate, double *__restrict__ T_blk) {
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int c0 = blockIdx.x * kCgsRows + w * 4;
-  if (c0 >= k) return;
-  double acc[4][kLP];
-#pragma unroll
-  for (int a = 0; a < 4; ++a)
-#pragma unroll
-    for (int q = 0; q < kLP; ++q) acc[a][q] = 0.0;
-  for (int i = lane; i < d; i += 64) {
-    double wv[kLP];
-#pragma unroll
-    for (int q = 0; q < kLP; ++q) wv[q] = W[(int64_t)q * d + i];
-#pragma unroll
-    for (int a = 0; a < 4; ++a) {
-      const double v = (c0 + a < k) ? V[(int64_t)(c0 + a) * d + i] : 0.0;
-#pragma unroll
-      for (int q = 0; q < kLP; ++q) acc[a][q] += v * wv[q];
-    }
-  }
-#pragma unroll
-  for (int a = 0; a < 4; ++a)
-#pragma unroll
-    for (int q = 0; q < kLP; ++q) {
-      const double s = wave_sum_f64(acc[a][q]);
-      const int c = c0 + a;
-      if (lane == 0 && c < k) {
-        H[(int64_t)c * kLP + q] = s;
-        if (c / kLP == jb) {
-          double *t = T_blk + (c - jb * kLP) * kLP + q;
-          *t = accumulate ? *t + s : s;
-        }
-      }
-    }
-}
-
-// part[s][q][i] = sum_{c in segment s} V[c][i] H[c][q]
-__global__ void __launch_bounds__(256) k_cgs_update_part(const double *__restrict__ V, int32_t k,
-                                                         int32_t d, const double *__restrict__ H,
-                                                         int32_t seg,
-                                                         double *__restrict__ part) {
-  __shared__ double hs[kSymSeg][kLP];
-  const int s = blockIdx.y;
-  const int c0 = s * seg, c1 = min(k, c0 + seg);
-  for (int e = threadIdx.x; e < (c1 - c0) * kLP; e += 256) hs[e / kLP][e % kLP] = H[(int64_t)c0 * kLP + e];
-  __syncthreads();
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= d) return;
+  const int lane = threadIdx.x & 63;
+  const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (c >= k) return;
   double acc[kLP];
 #pragma unroll
   for (int q = 0; q < kLP; ++q) acc[q] = 0.0;
-  for (int c = c0; c < c1; ++c) {
-    const double v = V[(int64_t)c * d + i];
+  const double *vc = V + (int64_t)c * d;
+  for (int i = lane; i < d; i += 64) {
+    const double v = vc[i];
 #pragma unroll
-    for (int q = 0; q < kLP; ++q) acc[q] += v * hs[c - c0][q];
+    for (int q = 0; q < kLP; ++q) acc[q] += v * W[(int64_t)q * d + i];
   }
 #pragma unroll
-  for (int q = 0; q < kLP; ++q) part[((int64_t)s * kLP + q) * d + i] = acc[q];
+  for (int q = 0; q < kLP; ++q) {
+    const double s = wave_sum_f64(acc[q]);
+    if (lane == 0) {
+      H[(int64_t)c * kLP + q] = s;
+      if (c / kLP == jb) {
+        double *t = T_blk + (c - jb * kLP) * kLP + q;
+        *t = accumulate ? *t + s : s;
+      }
+    }
+  }
 }
 
-// Upper Cholesky factor R (S = R^T R) of an 8 x 8 SPD matrix and its inverse.
-// Returns false when a pivot is not safely positive (condition of W beyond ~1e7).
-__device__ bool chol8(const double S[kLP][kLP], double R[kLP][kLP], double Ri[kLP][kLP]) {
+// W[q][i] -= sum_{c < k} V[c][i] H[c][q]: a workgroup owns 64 features; its 4 waves take
+// every 4th basis vector and their partial sums are added in a fixed order (deterministic).
+__global__ void __launch_bounds__(256) k_cgs_update(const double *__restrict__ V, int32_t k,
+                                                    int32_t d, const double *__restrict__ H,
+                                                    double *__restrict__ W) {
+  __shared__ double part[4][kLP][64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int i = blockIdx.x * 64 + lane;
+  double acc[kLP];
+#pragma unroll
+  for (int q = 0; q < kLP; ++q) acc[q] = 0.0;
+  if (i < d) {
+    for (int c = w; c < k; c += 4) {
+      const double v = V[(int64_t)c * d + i];
+#pragma unroll
+      for (int q = 0; q < kLP; ++q) acc[q] += v * H[(int64_t)c * kLP + q];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < kLP; ++q) part[w][q][lane] = acc[q];
+  __syncthreads();
+  // 256 threads finish 8 x 64 outputs: thread t -> q = t / 32 (two per thread)
+  for (int e = threadIdx.x; e < kLP * 64; e += 256) {
+    const int q = e >> 6, l = e & 63;
+    const int ii = blockIdx.x * 64 + l;
+    if (ii < d) {
+      const double s = ((part[0][q][l] + part[1][q][l]) + part[2][q][l]) + part[3][q][l];
+      W[(int64_t)q * d + ii] -= s;
+    }
+  }
+}
+
+// In-place upper Cholesky factor of an 8 x 8 SPD matrix held in registers (A = S on entry,
+// R with S = R^T R on exit; the strict lower part is left as is).  False when a pivot is
+// not safely positive (condition of W beyond ~1e7).
+__device__ __forceinline__ bool chol8_inplace(double (&A)[kLP][kLP]) {
   double smax = 0.0;
-  for (int q = 0; q < kLP; ++q) smax = fmax(smax, S[q][q]);
-  if (!(smax > 0.0)) return false;
-  for (int r = 0; r < kLP; ++r)
-    for (int c = 0; c < kLP; ++c) R[r][c] = 0.0;
+#pragma unroll
+  for (int q = 0; q < kLP; ++q) smax = fmax(smax, A[q][q]);
+  bool ok = smax > 0.0;
+#pragma unroll
   for (int r = 0; r < kLP; ++r) {
-    double dg = S[r][r];
-    for (int t = 0; t < r; ++t) dg -= R[t][r] * R[t][r];
-    if (!(dg > 1e-14 * smax)) return false;
-    const double p = sqrt(dg);
-    R[r][r] = p;
+    double dg = A[r][r];
+#pragma unroll
+    for (int t = 0; t < r; ++t) dg -= A[t][r] * A[t][r];
+    ok = ok && dg > 1e-14 * smax;
+    const double p = sqrt(fmax(dg, 1e-300));
+    A[r][r] = p;
+#pragma unroll
     for (int c = r + 1; c < kLP; ++c) {
-      double v = S[r][c];
-      for (int t = 0; t < r; ++t) v -= R[t][r] * R[t][c];
-      R[r][c] = v / p;
+      double v = A[r][c];
+#pragma unroll
+      for (int t = 0; t < r; ++t) v -= A[t][r] * A[t][c];
+      A[r][c] = v / p;
     }
   }
-  // inverse of an upper-triangular matrix, column by column
-  for (int r = 0; r < kLP; ++r)
-    for (int c = 0; c < kLP; ++c) Ri[r][c] = 0.0;
-  for (int c = 0; c < kLP; ++c) {
-    Ri[c][c] = 1.0 / R[c][c];
-    for (int r = c - 1; r >= 0; --r) {
-      double v = 0.0;
-      for (int t = r + 1; t <= c; ++t) v += R[r][t] * Ri[t][c];
-      Ri[r][c] = -v / R[r][r];
-    }
-  }
-  return true;
+  return ok;
 }
 
 // CholeskyQR2 of W [8][d] in one workgroup: Vout R = W with Vout orthonormal rows.
@@ -265,15 +252,16 @@ __global__ void __launch_bounds__(256) k_cholqr2(const double *__restrict__ W, i
                                                  double *__restrict__ Vout,
                                                  double *__restrict__ R_out,
                                                  int32_t *__restrict__ status) {
-  __shared__ double red[4][kLP * (kLP + 1) / 2];
-  __shared__ double S[kLP][kLP], R[kLP][kLP], Ri[kLP][kLP], Rt[kLP][kLP];
-  __shared__ int ok;
+  constexpr int NE = kLP * (kLP + 1) / 2;
+  __shared__ double red[4][NE];
+  __shared__ double Ri_s[kLP][kLP], Rt_s[kLP][kLP];
+  __shared__ int ok_s;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   for (int pass = 0; pass < 2; ++pass) {
     const double *src = pass == 0 ? W : Vout;
-    double acc[kLP * (kLP + 1) / 2];
+    double acc[NE];
 #pragma unroll
-    for (int e = 0; e < kLP * (kLP + 1) / 2; ++e) acc[e] = 0.0;
+    for (int e = 0; e < NE; ++e) acc[e] = 0.0;
     for (int i = tid; i < d; i += 256) {
       double x[kLP];
 #pragma unroll
@@ -285,42 +273,71 @@ __global__ void __launch_bounds__(256) k_cholqr2(const double *__restrict__ W, i
         for (int b = a; b < kLP; ++b) acc[e++] += x[a] * x[b];
     }
 #pragma unroll
-    for (int e = 0; e < kLP * (kLP + 1) / 2; ++e) {
+    for (int e = 0; e < NE; ++e) {
       const double s = wave_sum_f64(acc[e]);
       if (lane == 0) red[w][e] = s;
     }
     __syncthreads();
     if (tid == 0) {
+      double A[kLP][kLP];
       int e = 0;
+#pragma unroll
       for (int a = 0; a < kLP; ++a)
+#pragma unroll
         for (int b = a; b < kLP; ++b, ++e) {
-          const double v = red[0][e] + red[1][e] + red[2][e] + red[3][e];
-          S[a][b] = v;
-          S[b][a] = v;
+          const double v = ((red[0][e] + red[1][e]) + red[2][e]) + red[3][e];
+          A[a][b] = v;
+          A[b][a] = v;
         }
-      ok = chol8(S, R, Ri) ? 1 : 0;
-      if (ok) {
-        if (pass == 0) {
-          for (int a = 0; a < kLP; ++a)
-            for (int b = 0; b < kLP; ++b) Rt[a][b] = R[a][b];
-        } else {  // total factor R2 * R1 (both upper triangular)
-          double P[kLP][kLP];
-          for (int a = 0; a < kLP; ++a)
-            for (int b = 0; b < kLP; ++b) {
-              double v = 0.0;
-              for (int t = a; t <= b; ++t) v += R[a][t] * Rt[t][b];
-              P[a][b] = v;
-            }
-          for (int a = 0; a < kLP; ++a)
-            for (int b = 0; b < kLP; ++b) Rt[a][b] = P[a][b];
+      ok_s = chol8_inplace(A) ? 1 : 0;
+      // R^{-1} (upper triangular) column by column into LDS
+#pragma unroll
+      for (int c = 0; c < kLP; ++c) {
+        double col[kLP];
+#pragma unroll
+        for (int r = 0; r < kLP; ++r) col[r] = 0.0;
+        col[c] = 1.0 / A[c][c];
+#pragma unroll
+        for (int r = c - 1; r >= 0; --r) {
+          double v = 0.0;
+#pragma unroll
+          for (int t = r + 1; t <= c; ++t) v += A[r][t] * col[t];
+          col[r] = -v / A[r][r];
         }
+#pragma unroll
+        for (int r = 0; r < kLP; ++r) Ri_s[r][c] = col[r];
+      }
+      // accumulated factor: R1 after the first pass, R2 * R1 after the second (rows of
+      // the product only read rows >= their own of R1, so the update is in place)
+#pragma unroll
+      for (int a = 0; a < kLP; ++a) {
+        double row[kLP];
+#pragma unroll
+        for (int b = 0; b < kLP; ++b) {
+          double v = 0.0;
+          if (pass == 0) {
+            v = b >= a ? A[a][b] : 0.0;
+          } else {
+#pragma unroll
+            for (int t = 0; t < kLP; ++t)
+              if (t >= a && t <= b) v += A[a][t] * Rt_s[t][b];
+          }
+          row[b] = v;
+        }
+#pragma unroll
+        for (int b = 0; b < kLP; ++b) Rt_s[a][b] = row[b];
       }
     }
     __syncthreads();
-    if (!ok) {
+    if (!ok_s) {
       if (tid == 0) *status = 1;
       return;
     }
+    double Ri[kLP][kLP];
+#pragma unroll
+    for (int a = 0; a < kLP; ++a)
+#pragma unroll
+      for (int b = 0; b < kLP; ++b) Ri[a][b] = Ri_s[a][b];
     // rows of this thread: x <- x R^{-1}
     for (int i = tid; i < d; i += 256) {
       double x[kLP], y[kLP];
@@ -338,27 +355,7 @@ __global__ void __launch_bounds__(256) k_cholqr2(const double *__restrict__ W, i
     }
     __syncthreads();
   }
-  if (tid < kLP * kLP) R_out[tid] = Rt[tid / kLP][tid % kLP];
-}
-
-struct LanczosPlan {
-  int32_t sym_seg, sym_S;  // split-K of M V
-  int32_t cgs_seg, cgs_S;  // split of the basis in the update
-  size_t part_elems;       // doubles of the split-K partials
-};
-
-static LanczosPlan lanczos_plan(int32_t d, int32_t kmax) {
-  LanczosPlan p;
-  const int64_t bx = cdiv(d, 256);
-  // about 512 workgroups for M V, segments of at most kSymSeg rows
-  int64_t S = std::max<int64_t>(cdiv(d, kSymSeg), std::min<int64_t>(64, cdiv(512, bx)));
-  p.sym_seg = (int32_t)cdiv(d, S);
-  p.sym_S = (int32_t)cdiv(d, p.sym_seg);
-  int64_t Sc = std::max<int64_t>(cdiv(kmax, kSymSeg), std::min<int64_t>(64, cdiv(256, bx)));
-  p.cgs_seg = (int32_t)std::max<int64_t>(1, cdiv(kmax, Sc));
-  p.cgs_S = (int32_t)cdiv(kmax, p.cgs_seg);
-  p.part_elems = (size_t)std::max(p.sym_S, p.cgs_S) * kLP * d;
-  return p;
+  if (tid < kLP * kLP) R_out[tid] = Rt_s[tid / kLP][tid % kLP];
 }
 
 }  // namespace sh
@@ -395,9 +392,8 @@ extern "C" int sh_merge_matrix(const double *G, int32_t d, double n, const doubl
 
 extern "C" size_t sh_lanczos_workspace(int32_t d, int32_t kmax) {
   if (d <= 0 || kmax <= 0) return 0;
-  LanczosPlan p = lanczos_plan(d, kmax);
-  // W [8][d], H [kmax][8], partials
-  return align_up(((size_t)kLP * d + (size_t)kmax * kLP + p.part_elems) * sizeof(double), 256);
+  // W [8][d] and the pass coefficients H [kmax][8]
+  return align_up(((size_t)kLP * d + (size_t)kmax * kLP) * sizeof(double), 256);
 }
 
 extern "C" int sh_lanczos_steps(const double *M, int32_t d, double *V, int32_t kmax,
@@ -409,29 +405,19 @@ extern "C" int sh_lanczos_steps(const double *M, int32_t d, double *V, int32_t k
          "sh_lanczos_steps: steps exceed the basis capacity kmax");
   SH_ARG(ws_bytes >= sh_lanczos_workspace(d, kmax), "sh_lanczos_steps: workspace too small");
   hipStream_t st = (hipStream_t)stream;
-  LanczosPlan p = lanczos_plan(d, kmax);
   double *W = (double *)ws;
   double *H = W + (size_t)kLP * d;
-  double *part = H + (size_t)kmax * kLP;
-  const int64_t nW = (int64_t)kLP * d;
-  const dim3 bx((uint32_t)cdiv(d, 256));
   for (int32_t j = j0; j < j0 + n_steps; ++j) {
     const int32_t k = (j + 1) * kLP;  // basis vectors so far
     const double *Vj = V + (int64_t)j * kLP * d;
-    hipLaunchKernelGGL(k_symm_part, dim3(bx.x, p.sym_S), dim3(256), 0, st, M, d, Vj, p.sym_seg,
-                       part);
-    hipLaunchKernelGGL(k_sum_parts, dim3((uint32_t)cdiv(nW, 256)), dim3(256), 0, st, part,
-                       p.sym_S, nW, 1.0, 0, W);
+    hipLaunchKernelGGL(k_symm_rows, dim3((uint32_t)cdiv(d, 4 * kRowsPerWave)), dim3(256), 0,
+                       st, M, d, Vj, W);
     // two passes of classical Gram-Schmidt; each subtracts only its own coefficients
     for (int pass = 0; pass < 2; ++pass) {
-      hipLaunchKernelGGL(k_cgs_proj, dim3((uint32_t)cdiv(k, kCgsRows)), dim3(256), 0, st, V, k,
-                         d, W, H, j, pass, T_diag + (int64_t)j * kLP * kLP);
-      const int32_t seg = p.cgs_seg;
-      const int32_t S = (int32_t)cdiv(k, seg);
-      hipLaunchKernelGGL(k_cgs_update_part, dim3(bx.x, S), dim3(256), 0, st, V, k, d, H, seg,
-                         part);
-      hipLaunchKernelGGL(k_sum_parts, dim3((uint32_t)cdiv(nW, 256)), dim3(256), 0, st, part, S,
-                         nW, -1.0, 1, W);
+      hipLaunchKernelGGL(k_cgs_proj, dim3((uint32_t)cdiv(k, 4)), dim3(256), 0, st, V, k, d, W,
+                         H, j, pass, T_diag + (int64_t)j * kLP * kLP);
+      hipLaunchKernelGGL(k_cgs_update, dim3((uint32_t)cdiv(d, 64)), dim3(256), 0, st, V, k, d,
+                         H, W);
     }
     hipLaunchKernelGGL(k_cholqr2, dim3(1), dim3(256), 0, st, W, d, V + (int64_t)k * d,
                        T_sub + (int64_t)j * kLP * kLP, status);
