@@ -161,6 +161,7 @@ def main():
     ap.add_argument("--pca-mode", default=None, choices=["exact", "ipca"])
     ap.add_argument("--screen", default=None, choices=["bf16", "fp32", "fp8"])
     ap.add_argument("--recall-queries", type=int, default=None)
+    ap.add_argument("--pool", type=int, default=None, help="screen candidates per query")
     ap.add_argument("--compare-screen", default=None, choices=["bf16", "fp32", "fp8"],
                     help="also time the kNN stage with this screen on the same embedding")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -189,6 +190,8 @@ def main():
         wl["screen"] = args.screen
     if args.recall_queries:
         wl["recall_q"] = args.recall_queries
+    if args.pool:
+        wl["pool"] = args.pool
     torch.cuda.set_device(0)
     n, d = wl["n"], wl["d"]
     t0 = time.time()

@@ -122,8 +122,10 @@ int sh_merge_matrix(const double *G, int32_t d, double n, const double *bm,
 
 /* Block Lanczos (block size 8) with full two-pass classical Gram-Schmidt and
  * CholeskyQR2: runs block steps j0 .. j0 + n_steps - 1 on M (d x d symmetric).
- * V [kmax x d] holds the orthonormal basis, one vector per row; rows 0..7 are the
- * caller's starting block.  Step j writes rows 8(j+1)..8(j+1)+7, the diagonal block
+ * V [kmax x d] holds the orthonormal basis, one vector per row; rows below 8 (j0 + 1)
+ * come from the caller (the starting block, or after a thick restart the kept Ritz
+ * vectors followed by the newest block).  Step j orthogonalises M V_j against every
+ * row below 8 (j + 1) and writes rows 8(j+1)..8(j+1)+7, the diagonal block
  * T_diag[j] = V_j M V_j^T (8 x 8) and T_sub[j] = R_j (8 x 8 upper triangular, the block
  * below the diagonal of the Lanczos matrix).  status (i32, set to 0 by the caller)
  * becomes 1 when a new block is numerically rank deficient (basis exhausted).
@@ -195,6 +197,20 @@ int sh_knn_exact(const float *Q, int64_t nq, int64_t ldq, const int64_t *q_rows,
 int sh_fix_knn_query(const int64_t *idx_in, const float *dist_in, const int64_t *ref_idx,
                      int64_t n, int32_t k1, int64_t *idx_out, float *dist_out,
                      int32_t *flags_out, sh_stream_t stream);
+
+/* Locality order of embedding rows: a stable sort of the Morton code of the first three
+ * coordinates (10 bits each over their range).  perm_out [n] (i64) lists the rows in that
+ * order.  The screen takes its queries in this order so that the 32 queries of a tile are
+ * neighbours; candidate pools do not depend on query order, so results are unchanged.
+ * No counterpart in the reference (hnswlib queries one row at a time). */
+size_t sh_locality_order_workspace(int64_t n);
+int sh_locality_order(const float *E, int64_t n, int32_t D, int64_t lde, int64_t *perm_out,
+                      void *workspace, size_t workspace_bytes, sh_stream_t stream);
+/* Row permutation in 4-byte words: scatter == 0: dst[i] = src[perm[i]];
+ * scatter != 0: dst[perm[i]] = src[i].  Strides in bytes. */
+int sh_permute_rows(const void *src, int64_t src_ld_bytes, const int64_t *perm, int64_t n,
+                    int64_t row_bytes, void *dst, int64_t dst_ld_bytes, int scatter,
+                    sh_stream_t stream);
 
 /* ---- fuzzy edge weights ---------------------------------------------------------
  * umap smooth_knn_dist + compute_membership_strengths per `chunk_size` row chunk,

@@ -56,6 +56,9 @@ _SIGS = {
     "sh_knn_rerank": (_int, [_p, _i64, _i64, _p, _i64, _i64, _i32, _p, _p, _i32, _i32, _i32, _i64, _p, _f64, _f32, _p, _p, _p, _p]),
     "sh_knn_exact": (_int, [_p, _i64, _i64, _p, _i64, _p, _i64, _i64, _i32, _i32, _i64, _p, _p, _p, _p, _p, _p]),
     "sh_fix_knn_query": (_int, [_p, _p, _p, _i64, _i32, _p, _p, _p, _p]),
+    "sh_locality_order_workspace": (_sz, [_i64]),
+    "sh_locality_order": (_int, [_p, _i64, _i32, _i64, _p, _p, _sz, _p]),
+    "sh_permute_rows": (_int, [_p, _i64, _p, _i64, _i64, _p, _i64, _int, _p]),
     "sh_smooth_knn": (_int, [_p, _p, _i64, _i32, _i64, _f32, _f32, _i32, _p, _p, _p, _p, _sz, _p]),
     "sh_smooth_knn_workspace": (_sz, [_i64, _i64]),
     "sh_mapping_score": (_int, [_p, _p, _i64, _i32, _p, _i32, _i64, _int, _f64, _f64, _int, _p, _p, _sz, _p]),

@@ -197,96 +197,117 @@ struct OpFP8 {
   }
 };
 
-constexpr int kQG = 2;                 // 32-query groups per wave
-constexpr int kQPW = 32 * kQG;         // queries per wave
-constexpr int kQPB = 4 * kQPW;         // queries per workgroup
+
 constexpr int kSliceAlign = 128;       // reference slices start on this multiple
 constexpr float kOpen = FLT_MAX;       // threshold of a pool that is not full yet
 
+// QG: 32-query groups per wave; MINB: resident workgroups per CU the kernel is compiled
+// for (its register budget).  Two groups at three workgroups per CU measured faster than
+// four groups at two (BF16 K=64, 1M x 1M: 215 ms against 231 ms).
+template <class Op, int K>
+struct ScreenShape {
+  static constexpr int QG = 2;
+  static constexpr int QPB = 4 * 32 * QG;  // queries per workgroup
+  static constexpr int MINB = 3;
+};
+
+// Candidate buffer of one (query, slice): CAP keys in two halves of CAP / 2, one per lane
+// half (lanes n and n + 32 both serve query n); each lane appends to its own half with a
+// register counter, so inserting needs no atomics.  Compaction (the whole wave, for one
+// query) sorts both halves' c0 + c1 keys, keeps the best `pool` split between the two
+// halves (or writes them to `out`), and returns the new threshold: the pool-th key once
+// the query has seen `pool` candidates, else kOpen.
 template <int CAP>
-__device__ __forceinline__ void compact_query(uint64_t *qbuf, int *cnt_slot, float *tau_slot,
-                                              int pool, uint64_t *out, int32_t *out_cnt) {
+__device__ __forceinline__ int compact_pair(uint64_t *base, int c0, int c1, int pool,
+                                            uint64_t *out, int32_t *out_cnt, float &tau_new) {
   constexpr int E = CAP / 64;
+  constexpr int HALF = CAP / 2;
   const int lane = threadIdx.x & 63;
-  const int c = *cnt_slot;  // uniform
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
   uint64_t v[E];
 #pragma unroll
   for (int i = 0; i < E; ++i) {
     const int e = lane * E + i;
-    v[i] = e < c ? qbuf[e] : ~0ull;
+    v[i] = e < c0 ? base[e] : (e - c0 < c1 ? base[HALF + (e - c0)] : ~0ull);
   }
   wave_bitonic_sort<E>(v);
-  const int keep = c < pool ? c : pool;
-  uint64_t *dst = out ? out : qbuf;
+  const int total = c0 + c1;
+  const int keep = total < pool ? total : pool;
+  const int k0 = (keep + 1) / 2;
 #pragma unroll
   for (int i = 0; i < E; ++i) {
     const int e = lane * E + i;
-    if (e < keep) dst[e] = v[i];
+    if (e < keep) {
+      if (out)
+        out[e] = v[i];
+      else
+        base[e < k0 ? e : HALF + (e - k0)] = v[i];
+    }
   }
   const uint64_t kp = wave_elem_u64<E>(v, pool - 1);
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-  if (lane == 0) {
-    *cnt_slot = keep;
-    *tau_slot = c >= pool ? ord_float((uint32_t)(kp >> 32)) : kOpen;
-    if (out_cnt) *out_cnt = keep;
-  }
-  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+  if (out_cnt && lane == 0) *out_cnt = keep;
+  tau_new = total >= pool ? ord_float((uint32_t)(kp >> 32)) : kOpen;
+  return keep;
 }
 
-// Appends the values of one 4-reference quad of accumulator group g that pass tau.
+// Appends this lane's values of one 4-reference quad that pass tau to its buffer half.
 template <int V0>
-__device__ __forceinline__ void insert_quad(const floatx16 &acc, float tau, bool qv, int64_t rb,
-                                           int h, int *cnt, uint64_t *qbuf) {
+__device__ __forceinline__ void append_quad(const floatx16 &acc, float tau, bool qv, int64_t rb,
+                                            int h, uint64_t *half, int &cnt) {
 #pragma unroll
   for (int v = V0; v < V0 + 4; ++v) {
     if (qv && acc[v] <= tau) {
       const int64_t ref = rb + 8 * (v >> 2) + 4 * h + (v & 3);
-      const int pos = atomicAdd(cnt, 1);
-      qbuf[pos] = make_key(acc[v], (uint32_t)ref);
+      half[cnt] = make_key(acc[v], (uint32_t)ref);
+      ++cnt;
     }
   }
 }
 
 template <class Op, int K, int CAP>
-__global__ void __launch_bounds__(256) k_screen(
+__global__ void __launch_bounds__(256, (ScreenShape<Op, K>::MINB)) k_screen(
     const typename Op::Frag *__restrict__ Qp_, int64_t nq, const unsigned char *__restrict__ Rp,
     int64_t nr, int64_t slice_len, int S, int pool, uint64_t *__restrict__ buf,
     uint64_t *__restrict__ out, int32_t *__restrict__ out_cnt) {
   typedef typename Op::Frag Frag;
+  constexpr int QG = ScreenShape<Op, K>::QG;
+  constexpr int QPW = 32 * QG;                // queries per wave
+  constexpr int QPB = ScreenShape<Op, K>::QPB;
+  constexpr int HALFC = CAP / 2;              // keys per buffer half
   constexpr int HALF = K / 2;                 // elements per lane half
   constexpr int NS = HALF / Op::EPL;          // MFMA steps per tile
   constexpr int GROWB = K * Op::ESZ + (Op::kScaled ? 16 : 0);  // packed row bytes in HBM
   constexpr int ROWB = K * Op::ESZ + 16;      // LDS row bytes (pad or FP8 trailer)
   constexpr int CHUNKS = GROWB / 16;          // 16-byte chunks per packed row
-  // references per LDS stage: as many as two stage buffers fit in 60 KiB
-  constexpr int RT = (2 * 128 * ROWB <= 61440) ? 128 : (2 * 64 * ROWB <= 61440) ? 64 : 32;
+  // references per LDS stage: as many as two stage buffers fit in 48 KiB
+  constexpr int RT = (2 * 128 * ROWB <= 49152) ? 128 : (2 * 64 * ROWB <= 49152) ? 64 : 32;
   // stage buffers are filled by global_load_lds_dwordx4: each wave instruction writes
   // 64 consecutive 16-byte chunks (1 KiB), so a buffer is a whole number of KiB
   constexpr int CPR = ROWB / 16;                      // chunks per padded LDS row
   constexpr int NINSTR = (RT * CPR + 63) / 64;        // wave instructions per stage
   constexpr int BUFB = NINSTR * 1024;
   __shared__ __attribute__((aligned(1024))) unsigned char lds_r[2][BUFB];
-  __shared__ int cnt_lds[kQPB];
-  __shared__ float tau_lds[kQPB];
 
   const unsigned char *Qp = reinterpret_cast<const unsigned char *>(Qp_);
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, h = lane >> 5, n = lane & 31;
   const int slice = blockIdx.y;
   const int64_t r_begin = (int64_t)slice * slice_len;
   const int64_t r_end = min(nr, r_begin + slice_len);
-  const int qw0 = w * kQPW;  // wave's first local query
-  const int64_t qb0 = (int64_t)blockIdx.x * kQPB;
+  const int qw0 = w * QPW;  // wave's first local query
+  const int64_t qb0 = (int64_t)blockIdx.x * QPB;
 
-  Frag bq[kQG][NS];
-  float tau[kQG];
-  bool qvalid[kQG];
-  int qscale[kQG];
+  Frag bq[QG][NS];
+  float tau[QG];
+  bool qvalid[QG];
+  int qscale[QG];
+  int cnt[QG];
 #pragma unroll
-  for (int g = 0; g < kQG; ++g) {
+  for (int g = 0; g < QG; ++g) {
     const int64_t q = qb0 + qw0 + g * 32 + n;
     qvalid[g] = q < nq;
     tau[g] = kOpen;
+    cnt[g] = 0;
     qscale[g] = 127;
 #pragma unroll
     for (int s = 0; s < NS; ++s)
@@ -295,13 +316,6 @@ __global__ void __launch_bounds__(256) k_screen(
                            : Frag{};
     if (Op::kScaled && qvalid[g])
       qscale[g] = *reinterpret_cast<const int32_t *>(Qp + q * GROWB + K * Op::ESZ + 4);
-  }
-  if (h == 0) {
-#pragma unroll
-    for (int g = 0; g < kQG; ++g) {
-      cnt_lds[qw0 + g * 32 + n] = 0;
-      tau_lds[qw0 + g * 32 + n] = kOpen;
-    }
   }
 
   // double buffering: stage t+1 streams into the other buffer (LDS DMA) while stage t
@@ -335,7 +349,7 @@ __global__ void __launch_bounds__(256) k_screen(
     for (int sub = 0; sub < RT / 32; ++sub) {
       const int64_t rb = t0 + sub * 32;
       if (rb >= r_end) continue;
-      floatx16 acc[kQG];
+      floatx16 acc[QG];
       if constexpr (Op::kScaled) {
         // accumulator starts at ||r||^2 of each of the lane's 16 references
         floatx16 nrm;
@@ -344,10 +358,10 @@ __global__ void __launch_bounds__(256) k_screen(
           nrm[v] = *reinterpret_cast<const float *>(
               &lds_r[b][(sub * 32 + 8 * (v >> 2) + 4 * h + (v & 3)) * ROWB + K * Op::ESZ]);
 #pragma unroll
-        for (int g = 0; g < kQG; ++g) acc[g] = nrm;
+        for (int g = 0; g < QG; ++g) acc[g] = nrm;
       } else {
 #pragma unroll
-        for (int g = 0; g < kQG; ++g)
+        for (int g = 0; g < QG; ++g)
 #pragma unroll
           for (int v = 0; v < 16; ++v) acc[g][v] = 0.f;
       }
@@ -359,7 +373,7 @@ __global__ void __launch_bounds__(256) k_screen(
       for (int s = 0; s < NS; ++s) {
         Frag a = *reinterpret_cast<const Frag *>(arow + s * Op::EPL * Op::ESZ);
 #pragma unroll
-        for (int g = 0; g < kQG; ++g) {
+        for (int g = 0; g < QG; ++g) {
           if constexpr (Op::kScaled)
             acc[g] = Op::mma_s(a, bq[g][s], acc[g], ascale, qscale[g]);
           else
@@ -368,7 +382,7 @@ __global__ void __launch_bounds__(256) k_screen(
       }
       if (rb + 32 > r_end) {  // references past the slice end never pass
 #pragma unroll
-        for (int g = 0; g < kQG; ++g)
+        for (int g = 0; g < QG; ++g)
 #pragma unroll
           for (int v = 0; v < 16; ++v) {
             const int64_t ref = rb + 8 * (v >> 2) + 4 * h + (v & 3);
@@ -376,11 +390,11 @@ __global__ void __launch_bounds__(256) k_screen(
           }
       }
       // quad minima, then the tile minimum, per query group
-      float m4[kQG][4];
-      bool hit[kQG];
+      float m4[QG][4];
+      bool hit[QG];
       bool any_hit = false;
 #pragma unroll
-      for (int g = 0; g < kQG; ++g) {
+      for (int g = 0; g < QG; ++g) {
 #pragma unroll
         for (int j = 0; j < 4; ++j)
           m4[g][j] = fminf(fminf(acc[g][4 * j], acc[g][4 * j + 1]),
@@ -391,32 +405,35 @@ __global__ void __launch_bounds__(256) k_screen(
       }
       if (__any(any_hit)) {
 #pragma unroll
-        for (int g = 0; g < kQG; ++g) {
+        for (int g = 0; g < QG; ++g) {
           if (!__any(hit[g])) continue;
-          const int ql = qw0 + g * 32 + n;
-          const int64_t q = qb0 + ql;
-          uint64_t *qbuf = buf + ((qvalid[g] ? q : 0) * S + slice) * CAP;
+          const int64_t q = qb0 + qw0 + g * 32 + n;
+          uint64_t *half = buf + ((qvalid[g] ? q : 0) * S + slice) * CAP + h * HALFC;
           if (__any(hit[g] && m4[g][0] <= tau[g]))
-            insert_quad<0>(acc[g], tau[g], qvalid[g], rb, h, &cnt_lds[ql], qbuf);
+            append_quad<0>(acc[g], tau[g], qvalid[g], rb, h, half, cnt[g]);
           if (__any(hit[g] && m4[g][1] <= tau[g]))
-            insert_quad<4>(acc[g], tau[g], qvalid[g], rb, h, &cnt_lds[ql], qbuf);
+            append_quad<4>(acc[g], tau[g], qvalid[g], rb, h, half, cnt[g]);
           if (__any(hit[g] && m4[g][2] <= tau[g]))
-            insert_quad<8>(acc[g], tau[g], qvalid[g], rb, h, &cnt_lds[ql], qbuf);
+            append_quad<8>(acc[g], tau[g], qvalid[g], rb, h, half, cnt[g]);
           if (__any(hit[g] && m4[g][3] <= tau[g]))
-            insert_quad<12>(acc[g], tau[g], qvalid[g], rb, h, &cnt_lds[ql], qbuf);
-          __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-          // compact every query of this group whose buffer cannot take another tile
-          const bool need = cnt_lds[ql] > CAP - 32;
-          uint64_t mk = __ballot(need) & 0xffffffffull;  // lanes n and n+32 share a query
-          while (mk) {
-            const int src = __ffsll((unsigned long long)mk) - 1;
-            mk &= mk - 1;
-            const int qls = qw0 + g * 32 + src;
-            const int64_t qs = qb0 + qls;
-            compact_query<CAP>(buf + (qs * S + slice) * CAP, &cnt_lds[qls], &tau_lds[qls],
-                               pool, nullptr, nullptr);
+            append_quad<12>(acc[g], tau[g], qvalid[g], rb, h, half, cnt[g]);
+          // compact every query of the group one of whose halves cannot take a tile
+          const uint64_t mk = __ballot(cnt[g] > HALFC - 16);
+          uint32_t mq = (uint32_t)(mk | (mk >> 32));
+          while (mq) {
+            const int src = __ffs(mq) - 1;
+            mq &= mq - 1;
+            const int c0 = __builtin_amdgcn_readlane(cnt[g], src);
+            const int c1 = __builtin_amdgcn_readlane(cnt[g], src + 32);
+            const int64_t qs = qb0 + qw0 + g * 32 + src;
+            float tnew;
+            const int keep = compact_pair<CAP>(buf + (qs * S + slice) * CAP, c0, c1, pool,
+                                               nullptr, nullptr, tnew);
+            if (n == src) {
+              cnt[g] = h == 0 ? (keep + 1) / 2 : keep - (keep + 1) / 2;
+              tau[g] = tnew;
+            }
           }
-          tau[g] = tau_lds[ql];
         }
       }
     }
@@ -426,13 +443,17 @@ __global__ void __launch_bounds__(256) k_screen(
   }
 #undef SH_SCREEN_FETCH
   // final: sort each query's buffer and emit its best `pool`
-  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-  for (int j = 0; j < kQPW; ++j) {
-    const int ql = qw0 + j;
-    const int64_t q = qb0 + ql;
-    if (q >= nq) break;
-    compact_query<CAP>(buf + (q * S + slice) * CAP, &cnt_lds[ql], &tau_lds[ql], pool,
-                       out + (q * S + slice) * pool, out_cnt + q * S + slice);
+#pragma unroll
+  for (int g = 0; g < QG; ++g) {
+    for (int src = 0; src < 32; ++src) {
+      const int64_t q = qb0 + qw0 + g * 32 + src;
+      if (q >= nq) break;
+      const int c0 = __builtin_amdgcn_readlane(cnt[g], src);
+      const int c1 = __builtin_amdgcn_readlane(cnt[g], src + 32);
+      float tnew;
+      compact_pair<CAP>(buf + (q * S + slice) * CAP, c0, c1, pool, out + (q * S + slice) * pool,
+                        out_cnt + q * S + slice, tnew);
+    }
   }
 }
 
@@ -791,9 +812,10 @@ extern "C" size_t sh_knn_screen_workspace(int64_t nq, int32_t pool, int32_t n_sl
 }
 
 template <class Op, int K, int CAP>
-static void launch_screen(dim3 grid, hipStream_t st, const void *qpack, int64_t nq,
+static void launch_screen(hipStream_t st, const void *qpack, int64_t nq,
                           const void *rpack, int64_t nr, int64_t slice_len, int S, int pool,
                           uint64_t *buf, uint64_t *out, int32_t *cnt) {
+  dim3 grid((uint32_t)cdiv(nq, ScreenShape<Op, K>::QPB), S);
   hipLaunchKernelGGL((k_screen<Op, K, CAP>), grid, dim3(256), 0, st,
                      (const typename Op::Frag *)qpack, nq, (const unsigned char *)rpack, nr,
                      slice_len, S, pool, buf, out, cnt);
@@ -823,17 +845,17 @@ extern "C" int sh_knn_screen(const void *qpack, int64_t nq, const void *rpack, i
   const int S = (int)cdiv(nr, slice_len);
   SH_ARG(S == n_slices, "sh_knn_screen: %d slices requested but %lld references give %d; use "
          "a slice count whose slices hold >= 128 references", n_slices, (long long)nr, S);
-  dim3 grid(cdiv(nq, kQPB), S);
+  SH_ARG(cdiv(nq, 128) <= 0x7fffffffll, "sh_knn_screen: too many queries");
   hipStream_t st = (hipStream_t)stream;
   uint64_t *buf = (uint64_t *)workspace;
   const bool big = pool > 64;
 #define SH_SCREEN_CASE(OP, KV)                                                               \
   if (K == KV) {                                                                             \
     if (big)                                                                                 \
-      launch_screen<OP, KV, 512>(grid, st, qpack, nq, rpack, nr, slice_len, S, pool, buf,    \
+      launch_screen<OP, KV, 512>(st, qpack, nq, rpack, nr, slice_len, S, pool, buf,          \
                                  cand_keys, cand_cnt);                                       \
     else                                                                                     \
-      launch_screen<OP, KV, 256>(grid, st, qpack, nq, rpack, nr, slice_len, S, pool, buf,    \
+      launch_screen<OP, KV, 256>(st, qpack, nq, rpack, nr, slice_len, S, pool, buf,          \
                                  cand_keys, cand_cnt);                                       \
     SH_LAUNCH_CHECK("k_screen");                                                             \
     return SH_OK;                                                                            \

@@ -119,15 +119,76 @@ def _fix_slices(nr: int, s: int) -> int:
     return 1
 
 
+# query batches at least this large are screened in locality order
+ORDER_MIN_QUERIES = 8192
+
+
+def locality_order(E: "torch.Tensor", stream=None) -> "torch.Tensor":
+    """Rows of ``E`` (FP32 [n, D] on device) in Morton order of their first three
+    coordinates (sh_locality_order), as an int64 permutation."""
+    n, D = E.shape
+    perm = torch.empty(n, dtype=torch.int64, device=E.device)
+    if n == 0:
+        return perm
+    buf, nbytes = workspace("order").get(_lib.lib().sh_locality_order_workspace(n))
+    _lib.call("sh_locality_order", ptr(E), n, D, E.stride(0), ptr(perm), ptr(buf), nbytes,
+              stream_handle(stream))
+    return perm
+
+
+def permute_rows(src: "torch.Tensor", perm: "torch.Tensor", scatter: bool = False,
+                 out: Optional["torch.Tensor"] = None, stream=None) -> "torch.Tensor":
+    """Row gather (``out[i] = src[perm[i]]``) or scatter (``out[perm[i]] = src[i]``) of a
+    2-D device tensor whose rows are a whole number of 4-byte words."""
+    n = int(perm.numel())
+    if src.dim() == 1:
+        src = src.view(-1, 1)
+    if out is None:
+        out = torch.empty((n if not scatter else src.shape[0],) + tuple(src.shape[1:]),
+                          dtype=src.dtype, device=src.device)
+    o2 = out.view(out.shape[0], -1)
+    es = src.element_size()
+    _lib.call("sh_permute_rows", ptr(src), src.stride(0) * es, ptr(perm), n,
+              src.shape[1] * es, ptr(o2), o2.stride(0) * es, int(scatter), stream_handle(stream))
+    return out
+
+
 def search(Q: "torch.Tensor", R: "torch.Tensor", refs: PackedRefs, k: int, pool: int = 64,
            self_offset: Optional[int] = None, n_slices: Optional[int] = None,
            certify: Optional[bool] = None, self_idx: Optional["torch.Tensor"] = None,
-           stream=None):
+           order: Optional[bool] = None, stream=None):
     """Top-k of each query row of ``Q`` against ``R`` (both FP32 [*, D] on device).
 
-    Returns (idx int64 [nq, k], dist f32 [nq, k], flags int32 [nq]).
+    Returns (idx int64 [nq, k], dist f32 [nq, k], flags int32 [nq]).  ``order`` (default:
+    batches of at least ORDER_MIN_QUERIES rows) screens the queries in locality order;
+    the results are the same either way.
     """
     require_gpu()
+    nq = Q.shape[0]
+    if order is None:
+        order = nq >= ORDER_MIN_QUERIES
+    if not order or nq == 0:
+        return _search(Q, R, refs, k, pool, self_offset, n_slices, certify, self_idx, stream)
+    perm = locality_order(Q, stream)
+    Qs = permute_rows(Q, perm, stream=stream)
+    if self_idx is not None:
+        si = self_idx.to(device=Q.device, dtype=torch.int64).contiguous()
+        if si.numel() != nq:
+            raise ValueError("self_idx needs one reference index per query row")
+        si_s = permute_rows(si, perm, stream=stream).view(-1)
+    elif self_offset is not None:
+        si_s = perm + int(self_offset)
+    else:
+        si_s = None
+    idx_s, dist_s, flags_s = _search(Qs, R, refs, k, pool, None, n_slices, certify, si_s,
+                                     stream)
+    idx = permute_rows(idx_s, perm, scatter=True, stream=stream)
+    dist = permute_rows(dist_s, perm, scatter=True, stream=stream)
+    flags = permute_rows(flags_s.view(-1, 1), perm, scatter=True, stream=stream).view(-1)
+    return idx, dist, flags
+
+
+def _search(Q, R, refs, k, pool, self_offset, n_slices, certify, self_idx, stream):
     nq, D = Q.shape
     nr = R.shape[0]
     dev = Q.device

@@ -207,6 +207,25 @@ def test_knn_bf16_recall(cuda):
     np.testing.assert_array_equal(dist.cpu().numpy().ravel(), pd.astype(np.float32))
 
 
+@pytest.mark.parametrize("screen", ["fp32", "bf16", "fp8"])
+def test_knn_locality_order_invariant(cuda, screen):
+    """Screening the queries in locality order changes nothing: same indices, distances
+    and flags as the natural order (self query with duplicates, and a plain query)."""
+    import torch
+    from scarf_amd.knn import locality_order, pack_refs, search
+    R = _rand_embedding(30000, 50, 12, dup_rows=50)
+    Rt = torch.from_numpy(R).cuda()
+    refs = pack_refs(Rt, screen)
+    for so, Q in ((0, Rt), (None, Rt[::3])):
+        say("knn order", screen, "self" if so is not None else "plain", Q.shape[0])
+        a = search(Q, Rt, refs, 21, pool=64, self_offset=so, order=False)
+        b = search(Q, Rt, refs, 21, pool=64, self_offset=so, order=True)
+        for x, y in zip(a, b):
+            assert torch.equal(x, y)
+    perm = locality_order(Rt).cpu().numpy()
+    assert np.array_equal(np.sort(perm), np.arange(R.shape[0]))
+
+
 def test_knn_exact_kernel(cuda):
     import torch
     from scarf_amd.knn import exact_knn
