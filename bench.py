@@ -24,6 +24,9 @@ if ROOT not in sys.path:
 
 METRIC = "cells/s norm+PCA+kNN @1M cells, recall@11 vs FP64 oracle; mapping p50/p99 ms"
 PEAK = {"bf16": 2.5e15, "fp32": 157.3e12, "fp8": 5.0e15, "fp64": 78.6e12}
+# HBM bytes per screen launch from FETCH_SIZE / WRITE_SIZE counter passes scaled by a 1 GiB
+# copy (profiles/screen_bf16_1m_traffic.md), keyed by (workload, screen, pool)
+SCREEN_TRAFFIC = {("1m", "bf16", 64): 4.75e11}
 
 WORKLOADS = {
     # cells, genes, groups, density, chunk, dims, k, screen, pool, pca mode, recall queries
@@ -167,6 +170,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=20_000)
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="after the timed steps, copy 1 GiB device to device once (a kernel of "
+                         "known traffic that scales hardware counter readings)")
     ap.add_argument("--serving-seconds", type=float, default=12.0)
     args = ap.parse_args()
 
@@ -241,9 +247,19 @@ def main():
     op = {"bf16": "OpBF16", "fp32": "OpF32", "fp8": "OpFP8"}[wl["screen"]]
     roofline = {"kernel": f"k_screen<{op},{K},{cap}>", "bound": "mfma",
                 "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
-                "frac": achieved / peak, "traffic": None, "avg_launch_ms": avg_ms,
+                "frac": achieved / peak,
+                "traffic": SCREEN_TRAFFIC.get((args.workload, wl["screen"], wl["pool"])),
+                "avg_launch_ms": avg_ms,
                 "launches": len(scr_ms),
                 "work": f"2*nq*nr*D flops, D={wl['dims']} (unpadded), nq=nr={n}"}
+
+    if args.copy_probe:
+        x = torch.empty(1 << 27, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        y.copy_(x)
+        torch.cuda.synchronize()
+        del x, y
+        log("copy probe: 1 GiB read + 1 GiB written")
 
     rec = recall_vs_exact(res, wl["k"], n, wl["recall_q"])
     log(f"recall {rec}")
