@@ -251,15 +251,18 @@ __device__ __forceinline__ int compact_pair(uint64_t *base, int c0, int c1, int 
   return keep;
 }
 
+// True when the predicate holds on any lane of the wave (one ballot, no lane shuffles).
+__device__ __forceinline__ bool wave_any(bool p) { return __builtin_amdgcn_ballot_w64(p) != 0; }
+
 // Appends this lane's values of one 4-reference quad that pass tau to its buffer half.
+// Queries past the batch end carry tau = -inf, so nothing of theirs passes.
 template <int V0>
-__device__ __forceinline__ void append_quad(const floatx16 &acc, float tau, bool qv, int64_t rb,
-                                            int h, uint64_t *half, int &cnt) {
+__device__ __forceinline__ void append_quad(const floatx16 &acc, float tau, uint32_t ref0,
+                                            uint64_t *half, int &cnt) {
 #pragma unroll
   for (int v = V0; v < V0 + 4; ++v) {
-    if (qv && acc[v] <= tau) {
-      const int64_t ref = rb + 8 * (v >> 2) + 4 * h + (v & 3);
-      half[cnt] = make_key(acc[v], (uint32_t)ref);
+    if (acc[v] <= tau) {
+      half[cnt] = make_key(acc[v], ref0 + (uint32_t)(8 * (v >> 2) + (v & 3)));
       ++cnt;
     }
   }
@@ -306,7 +309,7 @@ __global__ void __launch_bounds__(256, (ScreenShape<Op, K>::MINB)) k_screen(
   for (int g = 0; g < QG; ++g) {
     const int64_t q = qb0 + qw0 + g * 32 + n;
     qvalid[g] = q < nq;
-    tau[g] = kOpen;
+    tau[g] = qvalid[g] ? kOpen : -INFINITY;  // a query past the batch end never passes
     cnt[g] = 0;
     qscale[g] = 127;
 #pragma unroll
@@ -400,23 +403,21 @@ __global__ void __launch_bounds__(256, (ScreenShape<Op, K>::MINB)) k_screen(
           m4[g][j] = fminf(fminf(acc[g][4 * j], acc[g][4 * j + 1]),
                            fminf(acc[g][4 * j + 2], acc[g][4 * j + 3]));
         const float m = fminf(fminf(m4[g][0], m4[g][1]), fminf(m4[g][2], m4[g][3]));
-        hit[g] = qvalid[g] && m <= tau[g];
+        hit[g] = m <= tau[g];
         any_hit |= hit[g];
       }
-      if (__any(any_hit)) {
+      if (wave_any(any_hit)) {
+        // lane's first reference of the tile (quads add 8 * (v / 4) + v % 4)
+        const uint32_t ref0 = (uint32_t)(rb + 4 * h);
 #pragma unroll
         for (int g = 0; g < QG; ++g) {
-          if (!__any(hit[g])) continue;
+          if (!wave_any(hit[g])) continue;
           const int64_t q = qb0 + qw0 + g * 32 + n;
           uint64_t *half = buf + ((qvalid[g] ? q : 0) * S + slice) * CAP + h * HALFC;
-          if (__any(hit[g] && m4[g][0] <= tau[g]))
-            append_quad<0>(acc[g], tau[g], qvalid[g], rb, h, half, cnt[g]);
-          if (__any(hit[g] && m4[g][1] <= tau[g]))
-            append_quad<4>(acc[g], tau[g], qvalid[g], rb, h, half, cnt[g]);
-          if (__any(hit[g] && m4[g][2] <= tau[g]))
-            append_quad<8>(acc[g], tau[g], qvalid[g], rb, h, half, cnt[g]);
-          if (__any(hit[g] && m4[g][3] <= tau[g]))
-            append_quad<12>(acc[g], tau[g], qvalid[g], rb, h, half, cnt[g]);
+          if (wave_any(m4[g][0] <= tau[g])) append_quad<0>(acc[g], tau[g], ref0, half, cnt[g]);
+          if (wave_any(m4[g][1] <= tau[g])) append_quad<4>(acc[g], tau[g], ref0, half, cnt[g]);
+          if (wave_any(m4[g][2] <= tau[g])) append_quad<8>(acc[g], tau[g], ref0, half, cnt[g]);
+          if (wave_any(m4[g][3] <= tau[g])) append_quad<12>(acc[g], tau[g], ref0, half, cnt[g]);
           // compact every query of the group one of whose halves cannot take a tile
           const uint64_t mk = __ballot(cnt[g] > HALFC - 16);
           uint32_t mq = (uint32_t)(mk | (mk >> 32));
