@@ -325,16 +325,42 @@ __global__ void __launch_bounds__(256, (ScreenShape<Op, K>::MINB)) k_screen(
   // computes.  Chunk ci of a stage lands at byte 16*ci: row ci / CPR, column chunk
   // ci % CPR; the padding chunk of each row and rows past the slice re-read a valid
   // address (their values are never used).
+  // Byte offset of this lane's chunk within a stage for each of its wave's instructions
+  // (instruction j = w + 4 i fills LDS bytes [1024 j, 1024 j + 1024)); the padding chunk
+  // of a row re-reads the row's first chunk (its value is never used).
+  constexpr int JPW = (NINSTR + 3) / 4;  // instructions per wave per stage (at most)
+  // rows the chunk layout touches per stage (can exceed RT by one when RT * CPR is not a
+  // multiple of 64)
+  constexpr int RROWS = (64 * NINSTR + CPR - 1) / CPR;
+  uint32_t choff[JPW];
+  int chrow[JPW];
+#pragma unroll
+  for (int i = 0; i < JPW; ++i) {
+    const int ci = 64 * (w + 4 * i) + lane;
+    const int rr = ci / CPR, c = ci % CPR;
+    chrow[i] = rr;
+    choff[i] = (uint32_t)(rr * GROWB + (c < CHUNKS ? c : 0) * 16);
+  }
 #define SH_SCREEN_FETCH(T0, B)                                                           \
-  _Pragma("unroll") for (int j = w; j < NINSTR; j += 4) {                                \
-    const int ci = 64 * j + lane;                                                        \
-    const int rr = ci / CPR, c = ci % CPR;                                               \
-    int64_t r = (T0) + rr;                                                               \
-    r = r < r_end ? r : r_end - 1;                                                       \
-    const unsigned char *src = Rp + r * GROWB + (c < CHUNKS ? c : 0) * 16;               \
-    __builtin_amdgcn_global_load_lds(                                                    \
-        (const void *)src,                                                               \
-        (__attribute__((address_space(3))) void *)(&lds_r[(B)][1024 * j]), 16, 0, 0);    \
+  {                                                                                      \
+    const unsigned char *stage = Rp + (T0) * GROWB;                                     \
+    const bool whole = (T0) + RROWS <= r_end; /* uniform: no row past the slice */       \
+    _Pragma("unroll") for (int i = 0; i < JPW; ++i) {                                    \
+      const int j = w + 4 * i;                                                           \
+      if (j < NINSTR) {                                                                  \
+        const unsigned char *src;                                                        \
+        if (whole) {                                                                     \
+          src = stage + choff[i];                                                        \
+        } else { /* rows past the slice re-read its last row */                          \
+          const int64_t last = r_end - 1 - (T0);                                         \
+          const int64_t over = chrow[i] > last ? (int64_t)chrow[i] - last : 0;           \
+          src = stage + choff[i] - over * GROWB;                                         \
+        }                                                                                \
+        __builtin_amdgcn_global_load_lds(                                                \
+            (const void *)src,                                                           \
+            (__attribute__((address_space(3))) void *)(&lds_r[(B)][1024 * j]), 16, 0, 0); \
+      }                                                                                  \
+    }                                                                                    \
   }
 
   int b = 0;
