@@ -43,9 +43,12 @@ COMMON_FLAGS = [
 
 
 # per-source extra flags: the kNN screen keeps MFMA accumulators in VGPRs (its filter
-# reads them every tile) and assumes no NaNs (INF is its only sentinel)
+# reads them every tile) and assumes no NaNs (INF is its only sentinel); the FP64 Gram
+# keeps them in VGPRs too (128 accumulator AGPRs on top of its VGPRs would leave one wave
+# per SIMD; in VGPRs it runs two)
 EXTRA_FLAGS = {
     "sh_knn.hip": ["-mllvm", "--amdgpu-mfma-vgpr-form", "-fno-honor-nans"],
+    "sh_gram.hip": ["-mllvm", "--amdgpu-mfma-vgpr-form"],
 }
 
 

@@ -352,9 +352,14 @@ class Projector:
                 z1 = np.clip(z1, -clip, clip)
             zc[fm] = z1[fm]
         c = zc @ L
-        t = lambda a, dt=torch.float32: torch.from_numpy(np.ascontiguousarray(a)).to(
-            device=device, dtype=dt)
-        return cls(t(L), t(mu), t(1.0 / sigma), t(c), float(clip) if clip else 0.0, D, d)
+        # one host buffer, one copy: [W (d x D) | mu (d) | 1/sigma (d) | c (D)] in FP32
+        host = np.concatenate([L.astype(np.float32).ravel(), mu.astype(np.float32),
+                               (1.0 / sigma).astype(np.float32), c.astype(np.float32)])
+        dev = torch.from_numpy(host).to(device=device)
+        W = dev[:d * D].view(d, D)
+        o = d * D
+        return cls(W, dev[o:o + d], dev[o + d:o + 2 * d], dev[o + 2 * d:o + 2 * d + D],
+                   float(clip) if clip else 0.0, D, d)
 
     def project(self, csr: DeviceCSR, params: NormParams, libsize=None, col_map=None,
                 row_start: int = 0, n_rows: Optional[int] = None, out=None, stream=None):
