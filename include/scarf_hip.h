@@ -236,6 +236,17 @@ int sh_mapping_score(const int64_t *idx, const float *dist, int64_t nq, int32_t 
                      size_t workspace_bytes, sh_stream_t stream);
 size_t sh_mapping_score_workspace(int64_t n_ref, int32_t n_groups);
 
+/* ---- label transfer ----------------------------------------------------------------
+ * Per target row of a mapping table (idx i64 [n x k], dist FP64 [n x k], as the
+ * projections store holds them): weights w = 1 - d / max(d over the row); the first k - 1
+ * neighbours vote with w for their reference class groups[idx] (i32 codes, [n_ref]); the
+ * row's class is the single class whose vote divided by the sum of the k - 1 weights
+ * (numpy's summation order) exceeds `threshold`, else -1.  k <= 129.
+ * Replaces: mapping_datastore.py:287-366 (get_target_classes). */
+int sh_target_classes(const int64_t *idx, const double *dist, int64_t n, int32_t k,
+                      const int32_t *groups, int64_t n_ref, double threshold, int32_t *out,
+                      sh_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

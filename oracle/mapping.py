@@ -62,3 +62,37 @@ def mapping_scores(indices, dists, ref_n_cells: int, target_groups=None,
             ms = np.log1p(ms)
         out.append((g, ms))
     return out
+
+
+def target_classes(indices, dists, ref_groups, threshold_fraction=0.5, target_subset=None,
+                   na_val="NA"):
+    """Label transfer, restating ``get_target_classes`` (``mapping_datastore.py:340-366``)
+    line for line: weights ``1 - d / max(d)`` per row, the first k - 1 neighbours vote in
+    neighbour order, the unique class above the threshold wins, else ``na_val``."""
+    indices = np.asarray(indices)
+    dists = np.asarray(dists, dtype=np.float64)
+    if target_subset is not None:
+        target_subset = {x: None for x in target_subset}
+    preds = []
+    with np.errstate(divide="ignore", invalid="ignore"):
+        weights = 1 - (dists / dists.max(axis=1).reshape(-1, 1))
+    for n in range(indices.shape[0]):
+        if target_subset is not None and n not in target_subset:
+            continue
+        wd = {}
+        for i, j in zip(indices[n, :-1], weights[n, :-1]):
+            k = ref_groups[i]
+            if k not in wd:
+                wd[k] = 0
+            wd[k] += j
+        temp = na_val
+        s = weights[n, :-1].sum()
+        for i, j in wd.items():
+            if j / s > threshold_fraction:
+                if temp == na_val:
+                    temp = i
+                else:
+                    temp = na_val
+                    break
+        preds.append(temp)
+    return preds

@@ -46,6 +46,82 @@ __global__ void k_ms_final(const uint64_t *__restrict__ acc, const int32_t *__re
   out[t] = log_t ? log1p(ms) : ms;
 }
 
+
+// ---- label transfer -------------------------------------------------------------------
+// numpy's float64 sum of a short row: sequential below 8 elements, else eight running
+// accumulators combined as ((r0+r1)+(r2+r3))+((r4+r5)+(r6+r7)) plus a sequential tail
+// (numpy's pairwise_sum for n <= 128), so a sum and its comparisons match numpy bit for bit.
+__device__ double numpy_sum(const double *v, int n) {
+  if (n < 8) {
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) s += v[i];
+    return s;
+  }
+  double r[8];
+  for (int j = 0; j < 8; ++j) r[j] = v[j];
+  int i = 8;
+  for (; i + 8 <= n; i += 8)
+    for (int j = 0; j < 8; ++j) r[j] += v[i + j];
+  double s = ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));
+  for (; i < n; ++i) s += v[i];
+  return s;
+}
+
+constexpr int kTcMaxK = 128;
+
+// One thread per target row: weights 1 - d / max(d) over the whole row, the first k - 1
+// neighbours vote with their weights for their reference class (classes in order of
+// first appearance, sums in neighbour order); the row's class is the single class whose
+// share of the vote exceeds `threshold`, else -1.
+__global__ void k_target_classes(const int64_t *__restrict__ idx, const double *__restrict__ dist,
+                                 int64_t n, int32_t k, const int32_t *__restrict__ groups,
+                                 int64_t n_ref, double threshold, int32_t *__restrict__ out) {
+  const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= n) return;
+  const int64_t *ir = idx + row * k;
+  const double *dr = dist + row * k;
+  double dmax = dr[0];  // numpy max: a NaN anywhere gives NaN
+  for (int j = 1; j < k; ++j) {
+    const double v = dr[j];
+    if (dmax == dmax && (v != v || v > dmax)) dmax = v;
+  }
+  double w[kTcMaxK];
+  int32_t cls[kTcMaxK];
+  double csum[kTcMaxK];
+  int nc = 0;
+  const int m = k - 1;
+  for (int j = 0; j < m; ++j) {
+    w[j] = 1.0 - dr[j] / dmax;
+    const int64_t r = ir[j];
+    const int32_t c = (r >= 0 && r < n_ref) ? groups[r] : -2;
+    int slot = -1;
+    for (int t = 0; t < nc; ++t)
+      if (cls[t] == c) {
+        slot = t;
+        break;
+      }
+    if (slot < 0) {
+      slot = nc++;
+      cls[slot] = c;
+      csum[slot] = 0.0;
+    }
+    csum[slot] += w[j];
+  }
+  const double s = numpy_sum(w, m);
+  int32_t res = -1;
+  for (int t = 0; t < nc; ++t) {
+    if (csum[t] / s > threshold) {
+      if (res == -1) {
+        res = cls[t];
+      } else {
+        res = -1;
+        break;
+      }
+    }
+  }
+  out[row] = res;
+}
+
 }  // namespace sh
 
 using namespace sh;
@@ -87,5 +163,18 @@ extern "C" int sh_mapping_score(const int64_t *idx, const float *dist, int64_t n
   hipLaunchKernelGGL(k_ms_final, dim3(cdiv(m, 256)), dim3(256), 0, st, acc, cnt, n_groups, n_ref,
                      k, multiplier, log_transform, out);
   SH_LAUNCH_CHECK("k_ms_final");
+  return SH_OK;
+}
+
+extern "C" int sh_target_classes(const int64_t *idx, const double *dist, int64_t n, int32_t k,
+                                 const int32_t *groups, int64_t n_ref, double threshold,
+                                 int32_t *out, sh_stream_t stream) {
+  SH_ARG(n >= 0 && k >= 2 && k <= kTcMaxK + 1 && n_ref >= 1,
+         "sh_target_classes: need n >= 0, 2 <= k <= %d, n_ref >= 1", kTcMaxK + 1);
+  if (n == 0) return SH_OK;
+  SH_ARG(idx && dist && groups && out, "sh_target_classes: NULL pointer");
+  hipLaunchKernelGGL(k_target_classes, dim3((uint32_t)cdiv(n, 128)), dim3(128), 0,
+                     (hipStream_t)stream, idx, dist, n, k, groups, n_ref, threshold, out);
+  SH_LAUNCH_CHECK("k_target_classes");
   return SH_OK;
 }

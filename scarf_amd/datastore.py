@@ -204,3 +204,61 @@ def get_mapping_score(store: Dict, target_name: str, target_groups=None,
                                 log_transform=log_transform, multiplier=multiplier,
                                 weighted=weighted, fixed_weight=fixed_weight):
         yield g, ms
+
+
+def get_target_classes(store: Dict, target_name: str, reference_class_group,
+                       threshold_fraction: float = 0.5, target_subset=None, na_val="NA",
+                       assay: str = "RNA"):
+    """Class of each target cell by weighted vote of its reference neighbours, like
+    ``get_target_classes`` (``mapping_datastore.py:287-366``); ``reference_class_group`` is
+    the reference cells' class labels (one per reference cell, as scarf fetches them from
+    the cell table).  Returns a pandas Series of labels, ``na_val`` where no class has more
+    than ``threshold_fraction`` of the vote."""
+    import pandas as pd
+    from .mapping import target_classes
+    loc = f"{assay}/projections/{target_name}"
+    if loc + "/indices" not in store:
+        raise KeyError(f"ERROR: Projections have not been computed for {target_name} in th "
+                       f"latest graph. Please run `run_mapping` or update latest_graph by "
+                       f"running `make_graph` with desired parameters")
+    if reference_class_group is None:
+        raise ValueError("ERROR: A value is required for the parameter "
+                         "`reference_class_group`.")
+    if threshold_fraction < 0 or threshold_fraction > 1:
+        raise ValueError("ERROR: `threshold_fraction` should have a value between 0 and 1")
+    if target_subset is not None and type(target_subset) != list:
+        raise TypeError("ERROR:  `target_subset` should be <list> type")
+    labels, codes = np.unique(np.asarray(reference_class_group), return_inverse=True)
+    idx = torch.from_numpy(np.asarray(store[loc + "/indices"]).astype(np.int64)).cuda()
+    dist = torch.from_numpy(np.asarray(store[loc + "/distances"], dtype=np.float64)).cuda()
+    res = target_classes(idx, dist, torch.from_numpy(codes.astype(np.int32)),
+                         threshold_fraction).cpu().numpy()
+    rows = range(res.shape[0])
+    if target_subset is not None:
+        keep = set(target_subset)
+        rows = [n for n in rows if n in keep]
+    return pd.Series([labels[res[n]] if res[n] >= 0 else na_val for n in rows])
+
+
+def load_unified_graph(store: Dict, target_names, use_k: int, target_weight: float,
+                       graph_loc: Optional[str] = None, assay: str = "RNA"):
+    """Reference kNN graph plus the first ``use_k`` edges of every projected cell of
+    ``target_names`` at a uniform ``target_weight``, as a CSR matrix
+    (``load_unified_graph``, ``mapping_datastore.py:368-412``).  Returns
+    (cell counts [reference, targets...], scipy CSR)."""
+    from scipy.sparse import csr_matrix
+    if graph_loc is None:
+        graph_loc = store["latest_graph"]
+    edges = np.asarray(store[graph_loc + "/edges"])
+    weights = np.asarray(store[graph_loc + "/weights"])
+    ref_n = int(store[f"{assay}/projections/{target_names[0]}/_ref_n_cells"])
+    pidx = np.vstack([np.asarray(store[f"{assay}/projections/{x}/indices"])[:, :use_k]
+                      for x in target_names])
+    n_cells = [ref_n] + [int(np.asarray(store[f"{assay}/projections/{x}/indices"]).shape[0])
+                         for x in target_names]
+    rows = (ref_n + np.repeat(np.arange(pidx.shape[0]), pidx.shape[1])).astype(np.int64)
+    ne = np.stack([rows, pidx.ravel().astype(np.int64)], axis=1)
+    me = np.vstack([edges.astype(np.int64), ne])
+    mw = np.hstack([weights, np.full(ne.shape[0], float(target_weight))])
+    tot = ref_n + pidx.shape[0]
+    return n_cells, csr_matrix((mw, (me[:, 0], me[:, 1])), shape=(tot, tot))

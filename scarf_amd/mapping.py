@@ -58,3 +58,20 @@ def mapping_scores(idx: "torch.Tensor", dist: "torch.Tensor", n_ref: int, groups
               int(log_transform), ptr(out), ptr(buf), nbytes, stream_handle(stream))
     o = out.cpu().numpy()
     return [(labels[i], o[i]) for i in range(ng)]
+
+
+def target_classes(idx: "torch.Tensor", dist: "torch.Tensor", ref_codes: "torch.Tensor",
+                   threshold: float = 0.5, stream=None) -> "torch.Tensor":
+    """Label transfer on the GPU (``sh_target_classes``): int32 class code per target row,
+    -1 where no single reference class holds more than ``threshold`` of the vote
+    (``get_target_classes``, ``mapping_datastore.py:287-366``).  ``dist`` is FP64, as the
+    projections store holds it; ``ref_codes`` int32 per reference cell."""
+    require_gpu()
+    n, k = idx.shape
+    out = torch.empty(max(n, 1), dtype=torch.int32, device=idx.device)
+    idx = idx.to(torch.int64).contiguous()
+    dist = dist.to(torch.float64).contiguous()
+    ref_codes = ref_codes.to(device=idx.device, dtype=torch.int32).contiguous()
+    _lib.call("sh_target_classes", ptr(idx), ptr(dist), n, k, ptr(ref_codes),
+              int(ref_codes.numel()), float(threshold), ptr(out), stream_handle(stream))
+    return out[:n]

@@ -117,3 +117,28 @@ def test_slice_plan():
             s = _fix_slices(nr, default_slices(nq, nr))
             sl = -(-(-(-nr // s)) // 128) * 128
             assert -(-nr // sl) == s
+
+
+def test_load_unified_graph_host():
+    """Reference edges plus the first use_k mapping edges of each target cell at a fixed
+    weight (mapping_datastore.py:368-412), against the reference's loop construction."""
+    from scipy.sparse import csr_matrix
+    from scarf_amd.datastore import load_unified_graph
+    rng = np.random.default_rng(7)
+    ref_n, k, nt, use_k = 40, 5, 13, 3
+    edges = np.stack([np.repeat(np.arange(ref_n), k), rng.integers(0, ref_n, ref_n * k)], 1)
+    weights = rng.random(ref_n * k)
+    pidx = rng.integers(0, ref_n, (nt, k)).astype(np.uint64)
+    store = {"latest_graph": "RNA/g", "RNA/g/edges": edges, "RNA/g/weights": weights,
+             "RNA/projections/t/indices": pidx, "RNA/projections/t/_ref_n_cells": ref_n}
+    n_cells, g = load_unified_graph(store, ["t"], use_k, 0.1)
+    ne, nw = [], []
+    for n, i in enumerate(pidx[:, :use_k]):
+        for j in i:
+            ne.append([ref_n + n, j])
+            nw.append(0.1)
+    me = np.vstack([edges, ne]).astype(int)
+    mw = np.hstack([weights, nw])
+    ref = csr_matrix((mw, (me[:, 0], me[:, 1])), shape=(ref_n + nt, ref_n + nt))
+    assert n_cells == [ref_n, nt]
+    assert (g != ref).nnz == 0 and g.shape == ref.shape

@@ -124,6 +124,20 @@ def test_run_mapping_and_scores(cuda, ref_data):
     (_, ms), = list(get_mapping_score(store, "t1"))
     (_, msr), = omap.mapping_scores(i_got, store["RNA/projections/t1/distances"], ann.nCells)
     np.testing.assert_allclose(ms, msr, rtol=1e-10, atol=1e-12)
+    # label transfer from the same table: reference classes from the first PC's sign and
+    # magnitude (4 classes), GPU vote vs the restated reference loop
+    from scarf_amd.datastore import get_target_classes
+    e0 = Ef[:, 0]
+    labels = np.where(e0 > np.median(e0), "hi", "lo").astype(object)
+    labels[np.abs(e0) < np.percentile(np.abs(e0), 10)] = "mid"
+    for thr in (0.5, 0.8):
+        say("get_target_classes threshold", thr)
+        pred = get_target_classes(store, "t1", labels, threshold_fraction=thr)
+        ref = omap.target_classes(i_got, store["RNA/projections/t1/distances"], labels, thr)
+        assert list(pred) == ref
+    sub = get_target_classes(store, "t1", labels, target_subset=[5, 1, 600])
+    assert list(sub) == omap.target_classes(i_got, store["RNA/projections/t1/distances"],
+                                            labels, 0.5, target_subset=[5, 1, 600])
 
 
 def test_target_projection_parity(cuda, ref_data):

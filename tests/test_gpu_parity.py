@@ -384,3 +384,28 @@ def test_knn_fp8_recall(cuda, D, pool):
     assert abs(recall - model) < 2e-3
     if D == 100:
         assert recall >= 0.99
+
+
+@pytest.mark.parametrize("k,threshold", [(12, 0.5), (22, 0.5), (12, 0.3), (6, 0.9), (2, 0.5)])
+def test_target_classes_gpu(cuda, k, threshold):
+    """GPU label transfer equals the restated reference loop row for row (FP64 weights,
+    numpy's summation order), including exact-threshold ties and zero-distance rows."""
+    import torch
+    from oracle.mapping import target_classes as ref_tc
+    from scarf_amd.mapping import target_classes
+    rng = np.random.default_rng(20 + k)
+    n, n_ref, ncls = 3000, 500, 7
+    groups = rng.integers(0, ncls, n_ref)
+    idx = rng.integers(0, n_ref, (n, k))
+    d = np.sort(rng.random((n, k)).astype(np.float32).astype(np.float64), axis=1)
+    # ties: rows whose neighbours split evenly between two classes at equal distance
+    d[:50] = 1.0
+    d[:50, -1] = 2.0
+    idx[:50, : k - 1] = np.where(np.arange(k - 1) % 2 == 0, 0, 1)
+    groups[0], groups[1] = 0, 1
+    d[50:60] = 0.0  # all-zero rows: NaN weights
+    say("target classes k", k, "threshold", threshold)
+    got = target_classes(torch.from_numpy(idx).cuda(), torch.from_numpy(d).cuda(),
+                         torch.from_numpy(groups.astype(np.int32)), threshold).cpu().numpy()
+    ref = ref_tc(idx, d, groups, threshold, na_val=-1)
+    np.testing.assert_array_equal(got, np.array(ref))

@@ -235,3 +235,25 @@ def test_align_features():
     out = mapping.align_normed(counts, t_idx, sf=10)
     # library size over present features only: 4 + 1 = 5
     np.testing.assert_allclose(out[0], [np.log1p(10 * 1 / 5), 1.0, np.log1p(10 * 4 / 5), 1.0])
+
+
+def test_target_classes_oracle_hand_cases():
+    """Label transfer (mapping_datastore.py:340-366) on rows worked by hand."""
+    from oracle.mapping import target_classes
+    groups = np.array(["A", "B", "A", "C", "B"])
+    idx = np.array([[0, 1, 2, 3],      # classes A B A | last column ignored
+                    [1, 4, 1, 0],      # B B B
+                    [0, 1, 3, 2],      # A B C
+                    [0, 0, 0, 0]])
+    d = np.array([[1.0, 2.0, 3.0, 4.0],   # w = .75 .5 .25 -> A 1.0, B .5 of 1.5
+                  [0.0, 1.0, 2.0, 2.0],   # w = 1 .5 0 -> B 1.5 of 1.5
+                  [1.0, 1.0, 1.0, 2.0],   # w = .5 .5 .5 -> thirds
+                  [0.0, 0.0, 0.0, 0.0]])  # max 0 -> NaN weights -> NA
+    assert target_classes(idx, d, groups, 0.5) == ["A", "B", "NA", "NA"]
+    assert target_classes(idx, d, groups, 0.7) == ["NA", "B", "NA", "NA"]
+    # two classes above a low threshold -> NA; one third is not above one third
+    assert target_classes(idx, d, groups, 0.3) == ["NA", "B", "NA", "NA"]
+    assert target_classes(idx, d, groups, 0.3333)[2] == "NA"
+    assert target_classes(idx, d, groups, 0.5, target_subset=[1, 3]) == ["B", "NA"]
+    # an exact half is not above one half
+    assert target_classes(np.array([[0, 1, 2]]), np.array([[1.0, 1.0, 2.0]]), groups, 0.5) == ["NA"]
