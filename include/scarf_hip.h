@@ -41,6 +41,8 @@ extern "C" {
 #define SH_F32 0
 #define SH_BF16 1
 #define SH_FP8 2
+/* Row type of the k-means entry points (SH_F32 rows are promoted to FP64 exactly). */
+#define SH_F64 3
 
 /* Bits of the per-query flags written by sh_knn_rerank / sh_knn_exact. */
 #define SH_FLAG_UNCERTIFIED 1 /* FP32 screen could not prove its top-k exact */
@@ -246,6 +248,43 @@ size_t sh_mapping_score_workspace(int64_t n_ref, int32_t n_groups);
 int sh_target_classes(const int64_t *idx, const double *dist, int64_t n, int32_t k,
                       const int32_t *groups, int64_t n_ref, double threshold, int32_t *out,
                       sh_stream_t stream);
+
+/* ---- MiniBatchKMeans seed partitions --------------------------------------------------
+ * scarf's AnnStream._fit_kmeans (ann.py:328-346) runs scikit-learn 1.7.2's
+ * MiniBatchKMeans: one partial_fit per block of cells (sklearn cluster/_kmeans.py:2203-2303),
+ * then predict on every block (_kmeans.py:1066-1098).  The host keeps the random state and
+ * makes the draws in scikit-learn's order; these entry points do the arithmetic in FP64.
+ * Rows X are FP32 or FP64 (x_dtype SH_F32 / SH_F64), row-major with leading dimension ldx;
+ * centers are FP64 [n_clusters x d], contiguous.
+ *
+ * sh_kmeans_pp: k-means++ seeding of n_clusters centers among the n points
+ * X[rows[i]] (or X[i] if rows is NULL), replacing _kmeans.py:174-272 (_kmeans_plusplus).
+ * `first` is the first center (the host's random_state.choice), `uniforms` the
+ * (n_clusters - 1) x n_trials draws of random_state.uniform in order.  Writes the chosen
+ * point indices (0..n-1) to centers_out.  d <= 256, n_trials <= 16; one workgroup. */
+size_t sh_kmeans_pp_workspace(int64_t n, int32_t n_trials);
+int sh_kmeans_pp(const void *X, int x_dtype, int64_t n, int32_t d, int64_t ldx,
+                 const int64_t *rows, int32_t n_clusters, int32_t n_trials, int64_t first,
+                 const double *uniforms, int64_t *centers_out, void *workspace,
+                 size_t workspace_bytes, sh_stream_t stream);
+/* labels[i] = argmin_j ||c_j||^2 - 2 x_i.c_j, the lowest j on ties; center_norms [n_clusters]
+ * receives ||c_j||^2.  Replaces _k_means_lloyd.pyx:172-215 as called by _labels_inertia
+ * (_kmeans.py:755-822) in partial_fit and predict. */
+int sh_kmeans_assign(const void *X, int x_dtype, int64_t n, int32_t d, int64_t ldx,
+                     const double *centers, int32_t n_clusters, double *center_norms,
+                     int32_t *labels, sh_stream_t stream);
+/* In-place minibatch update of every center with its members among the n rows (unit
+ * weights): c = (c * w + sum of members in row order) * (1 / (w + count)), w += count,
+ * unchanged when count is 0; counts[j] receives the member count.  Same roundings as
+ * _k_means_minibatch.pyx:59-108 (update_center_dense).  d <= 1024. */
+int sh_kmeans_update(const void *X, int x_dtype, int64_t n, int32_t d, int64_t ldx,
+                     const int32_t *labels, double *centers, int32_t n_clusters,
+                     double *weight_sums, int32_t *counts, sh_stream_t stream);
+/* centers[dst_rows[r]] = X[src_rows[r]] for r < m (dst_rows NULL: r).  Places the seeds
+ * (_kmeans.py:1003-1020) and the random reassignments (_kmeans.py:1642-1673). */
+int sh_kmeans_gather(const void *X, int x_dtype, int32_t d, int64_t ldx,
+                     const int64_t *src_rows, const int32_t *dst_rows, int64_t m,
+                     double *centers, sh_stream_t stream);
 
 #ifdef __cplusplus
 }

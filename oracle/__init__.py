@@ -2,7 +2,7 @@
 
 This package restates, in numpy and plain C, what the reference computes on the path
 normalise -> mu/sigma -> incremental PCA -> projection -> kNN -> smooth_knn_dist ->
-mapping scores.  Every function cites the reference file:line it follows (paths are
+mapping scores, and the MiniBatchKMeans seed partitions beside it.  Every function cites the reference file:line it follows (paths are
 relative to the scarf source tree).
 
 It exists to check the HIP implementation in ``scarf_amd`` and is never part of it:
@@ -26,4 +26,7 @@ Pinning (what each part is checked against, in ``tests/test_oracle.py``):
                 at the reference test's tolerance (1e-5), plus the zero-weight
                 global-minimum rule on a hand-built case.
 * ``mapping`` — hand-computed mapping scores and hand-built feature alignment.
+* ``kmeans``  — scikit-learn's ``MiniBatchKMeans`` and ``kmeans_plusplus`` (the
+                reference's seed-partition engine) on seeded inputs, bit for bit, plus a
+                hand-computed center-update chain.
 """

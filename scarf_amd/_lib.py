@@ -18,7 +18,7 @@ SH_EHIP = -2
 SH_EWORKSPACE = -3
 SH_EUNSUPPORTED = -4
 
-SH_F32, SH_BF16, SH_FP8 = 0, 1, 2
+SH_F32, SH_BF16, SH_FP8, SH_F64 = 0, 1, 2, 3
 DTYPES = {"fp32": SH_F32, "f32": SH_F32, "bf16": SH_BF16, "fp8": SH_FP8}
 
 SH_FLAG_UNCERTIFIED = 1
@@ -64,6 +64,11 @@ _SIGS = {
     "sh_mapping_score": (_int, [_p, _p, _i64, _i32, _p, _i32, _i64, _int, _f64, _f64, _int, _p, _p, _sz, _p]),
     "sh_mapping_score_workspace": (_sz, [_i64, _i32]),
     "sh_target_classes": (_int, [_p, _p, _i64, _i32, _p, _i64, _f64, _p, _p]),
+    "sh_kmeans_pp_workspace": (_sz, [_i64, _i32]),
+    "sh_kmeans_pp": (_int, [_p, _int, _i64, _i32, _i64, _p, _i32, _i32, _i64, _p, _p, _p, _sz, _p]),
+    "sh_kmeans_assign": (_int, [_p, _int, _i64, _i32, _i64, _p, _i32, _p, _p, _p]),
+    "sh_kmeans_update": (_int, [_p, _int, _i64, _i32, _i64, _p, _p, _i32, _p, _p, _p]),
+    "sh_kmeans_gather": (_int, [_p, _int, _i32, _i64, _p, _p, _i64, _p, _p]),
 }
 
 _lock = threading.Lock()

@@ -22,6 +22,7 @@ from typing import Optional
 import numpy as np
 
 from .device import DeviceCSR, col_map_tensor, ptr, require_gpu, stream_handle
+from .kmeans import MiniBatchKMeans
 from .knn import GpuIndex, fix_knn_query_device
 from .normalize import FeatureStats, NormParams, chunk_bounds, feature_stats
 from .pca import PcaFit, Projector, _z_params, fit_pca
@@ -221,10 +222,7 @@ class AnnStream:
             self.annIdx = ann_idx
             self.annIdx.set_ef(self.annEf)
             self.annIdx.set_num_threads(self.annThreads)
-        self.kmeans = None
-        if do_kmeans_fit:
-            logger.info("MiniBatchKMeans seed partitions are not built by scarf_amd; "
-                        "clusterLabels stay -1")
+        self.kmeans = self._fit_kmeans(do_kmeans_fit)
 
     # -- ann.py:173-185 ------------------------------------------------------------------
     def _handle_batch_size(self):
@@ -285,6 +283,22 @@ class AnnStream:
                       precision=gram_precision)
         self._pca = fit.state
         self.loadings = fit.loadings
+
+    # -- ann.py:328-346 ------------------------------------------------------------------
+    def _fit_kmeans(self, do_ann_fit):
+        """MiniBatchKMeans seed partitions: partial_fit on every block's embedding, then
+        predict on every block.  The embedding stays on the device; labels are int32 like
+        scikit-learn's.  ``labels_`` is not computed per step (scarf never reads it)."""
+        if do_ann_fit is False:
+            return None
+        km = MiniBatchKMeans(n_clusters=self.nClusters, random_state=self.randState,
+                             batch_size=self.batchSize, n_init=3, compute_labels=False)
+        for b in self.iter_blocks(msg="Fitting kmeans"):
+            km.partial_fit(self.reducer_tensor(b))
+        labels = [km.predict_tensor(self.reducer_tensor(b))
+                  for b in self.iter_blocks(msg="Estimating seed partitions")]
+        self.clusterLabels = torch.cat(labels).cpu().numpy()
+        return km
 
     # -- ann.py:290-326 ------------------------------------------------------------------
     def _fit_ann(self):

@@ -60,7 +60,7 @@ def make_graph(counts, feat_idx=None, *, assay: str = "RNA", cell_key: str = "I"
                feat_scaling: bool = True, custom_loadings: Optional[np.ndarray] = None,
                clip: Optional[float] = None, return_ann_object: bool = False,
                screen: str = "bf16", pool: int = 64, pca_mode: str = "ipca",
-               store: Optional[Dict] = None):
+               n_centroids: Optional[int] = None, store: Optional[Dict] = None):
     """``GraphDataStore.make_graph`` (``graph_datastore.py:513-1020``) on CSR counts.
 
     Parameter defaults follow ``_set_graph_params`` (``graph_datastore.py:139-149``,
@@ -92,6 +92,14 @@ def make_graph(counts, feat_idx=None, *, assay: str = "RNA", cell_key: str = "I"
     normed_loc, reduction_loc, ann_loc, knn_loc, graph_loc = graph_locations(
         assay, cell_key, feat_key, dims, pca_cell_key, ann_metric, ann_efc, ann_ef, ann_m,
         rand_state, k, local_connectivity, bandwidth)
+    # graph_datastore.py:307-321 and 744: the latest seed partitions' size, else 1000
+    if n_centroids is None:
+        latest = store.get(reduction_loc + "/latest_kmeans")
+        n_centroids = (int(latest.split("/")[-1].split("__")[1]) if latest
+                       else DEFAULTS["n_centroids"])
+    n_centroids = int(n_centroids)
+    kmeans_loc = f"{reduction_loc}/kmeans__{n_centroids}__{rand_state}"
+    fit_kmeans = kmeans_loc + "/cluster_centers" not in store
     st = data.stats()
     mu = store.get(normed_loc + "/mu")
     sigma = store.get(normed_loc + "/sigma")
@@ -107,16 +115,21 @@ def make_graph(counts, feat_idx=None, *, assay: str = "RNA", cell_key: str = "I"
     if custom_loadings is not None:
         loadings = np.asarray(custom_loadings, dtype=np.float64)
     ann_idx = store.get(ann_loc + "/ann_idx")
-    ann = AnnStream(data=data, k=k, n_cluster=DEFAULTS["n_centroids"],
+    ann = AnnStream(data=data, k=k, n_cluster=n_centroids,
                     reduction_method=method, dims=dims, loadings=loadings,
                     use_for_pca=use_for_pca, mu=mu, sigma=sigma, ann_metric=ann_metric,
                     ann_efc=ann_efc, ann_ef=ann_ef, ann_m=ann_m, nthreads=1,
-                    ann_parallel=False, rand_state=rand_state, do_kmeans_fit=False,
+                    ann_parallel=False, rand_state=rand_state, do_kmeans_fit=fit_kmeans,
                     disable_scaling=not feat_scaling, ann_idx=ann_idx, lsi_skip_first=True,
                     lsi_params={}, harmonize=False, screen=screen, pool=pool,
                     pca_mode=pca_mode)
     store[reduction_loc + "/reduction"] = ann.loadings
     store[ann_loc + "/ann_idx"] = ann.annIdx
+    if fit_kmeans:
+        # graph_datastore.py:958-975: centers and labels, both float64
+        store[kmeans_loc + "/cluster_centers"] = ann.kmeans.cluster_centers_.astype(np.float64)
+        store[kmeans_loc + "/cluster_labels"] = ann.clusterLabels.astype(np.float64)
+    store[reduction_loc + "/latest_kmeans"] = kmeans_loc
     if knn_loc + "/indices" not in store or graph_loc + "/weights" not in store:
         recall = None
         if knn_loc + "/indices" not in store:

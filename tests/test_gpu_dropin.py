@@ -64,6 +64,17 @@ def test_make_graph_chain(cuda, ref_data):
     np.testing.assert_array_equal(store[knn_loc + "/distances"],
                                   fd.astype(np.float32).astype(np.float64))
     assert store[knn_loc + "/recall"] == pytest.approx(rec)
+    # seed partitions: the oracle's MiniBatchKMeans on the same embedding, block by block
+    # (n_centroids 1000 -> 500 clusters, the batch size; graph_datastore.py:744, 958-975)
+    from oracle import kmeans as okm
+    kloc = [v for key, v in store.items() if key.endswith("/latest_kmeans")][0]
+    assert kloc.endswith("/kmeans__1000__4466")
+    km = okm.MiniBatchKMeans(n_clusters=500, random_state=4466, batch_size=500)
+    for s in range(0, E.shape[0], 500):
+        km.partial_fit(E[s:s + 500])
+    np.testing.assert_array_equal(store[kloc + "/cluster_centers"], km.cluster_centers_)
+    np.testing.assert_array_equal(store[kloc + "/cluster_labels"],
+                                  km.predict(E).astype(np.float64))
     # weights: oracle smoothing on the same kNN table
     e_ref, w_ref = osmooth.smoothen_dists(fi, fd.astype(np.float32), 1.0, 1.5, 500)
     g = store["latest_graph"]

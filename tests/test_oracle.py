@@ -257,3 +257,59 @@ def test_target_classes_oracle_hand_cases():
     assert target_classes(idx, d, groups, 0.5, target_subset=[1, 3]) == ["B", "NA"]
     # an exact half is not above one half
     assert target_classes(np.array([[0, 1, 2]]), np.array([[1.0, 1.0, 2.0]]), groups, 0.5) == ["NA"]
+
+
+# ---- MiniBatchKMeans seed partitions (ann.py:328-346) -------------------------------
+@pytest.mark.parametrize("case", ["blobs_k8", "small_blocks_k40", "init_subsample",
+                                  "explicit_init_size", "k_equals_first_block",
+                                  "outliers_cap"])
+def test_kmeans_oracle_matches_sklearn(case):
+    """The restated MiniBatchKMeans reproduces scikit-learn's seeds, labels, counts and
+    centers bit for bit, through the init subsample, empty-cluster reassignments and the
+    0.5 * batch reassignment cap."""
+    import warnings
+
+    from sklearn.cluster import MiniBatchKMeans
+    from kmeans_cases import CASES, blocks_of
+    from oracle import kmeans as okm
+
+    X, k, bs, block, init_size, first = CASES[case]()
+    ref = MiniBatchKMeans(n_clusters=k, random_state=7, batch_size=bs, n_init=3,
+                          init_size=init_size)
+    mine = okm.MiniBatchKMeans(n_clusters=k, random_state=7, batch_size=bs,
+                               init_size=init_size)
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        for b in blocks_of(X, block, first):
+            ref.partial_fit(b)
+            mine.partial_fit(b)
+    assert np.array_equal(ref._counts, mine._counts)
+    assert np.array_equal(ref.cluster_centers_, mine.cluster_centers_)
+    assert np.array_equal(ref.predict(X), mine.predict(X))
+    if case == "outliers_cap":
+        sizes = [len(c) for (_, c, _) in mine.reassign_log]
+        assert sizes and sizes[0] == 25  # 40 empty outlier centers, capped at 0.5 * 50
+
+
+def test_kmeans_plusplus_oracle_matches_sklearn():
+    from sklearn.cluster import kmeans_plusplus
+    from kmeans_cases import blobs
+    from oracle import kmeans as okm
+
+    X = blobs(9, 700, 6, 12)
+    _, want = kmeans_plusplus(X, 50, random_state=np.random.RandomState(3))
+    _, got = okm.kmeans_plusplus(X, 50, okm.row_norms_sq(X), np.random.RandomState(3))
+    assert np.array_equal(want, got)
+
+
+def test_kmeans_update_is_sequential_chain():
+    """c*w, + x in row order, * (1 / (w + n)): (0.1*3 + 0.2 + 0.7) * (1/5) differs from the
+    plain mean (0.3 + 0.2 + 0.7) / 5 in the last bit, and an empty center is untouched."""
+    from oracle import kmeans as okm
+
+    X = np.array([[0.2], [0.7], [5.0]])
+    C = np.array([[0.1], [9.0]])
+    w = np.array([3.0, 4.0])
+    okm.minibatch_update(X, C, w, np.array([0, 0, 0], dtype=np.int32))
+    assert C[0, 0] == ((0.1 * 3.0 + 0.2) + 0.7 + 5.0) * (1.0 / 6.0)
+    assert C[1, 0] == 9.0 and w.tolist() == [6.0, 4.0]
