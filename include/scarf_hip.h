@@ -269,10 +269,14 @@ int sh_kmeans_pp(const void *X, int x_dtype, int64_t n, int32_t d, int64_t ldx,
                  size_t workspace_bytes, sh_stream_t stream);
 /* labels[i] = argmin_j ||c_j||^2 - 2 x_i.c_j, the lowest j on ties; center_norms [n_clusters]
  * receives ||c_j||^2.  Replaces _k_means_lloyd.pyx:172-215 as called by _labels_inertia
- * (_kmeans.py:755-822) in partial_fit and predict. */
+ * (_kmeans.py:755-822) in partial_fit and predict.  With a workspace of
+ * sh_kmeans_assign_workspace bytes (0 for large n) few rows are spread over more
+ * workgroups; the labels do not depend on it. */
+size_t sh_kmeans_assign_workspace(int64_t n, int32_t n_clusters);
 int sh_kmeans_assign(const void *X, int x_dtype, int64_t n, int32_t d, int64_t ldx,
                      const double *centers, int32_t n_clusters, double *center_norms,
-                     int32_t *labels, sh_stream_t stream);
+                     int32_t *labels, void *workspace, size_t workspace_bytes,
+                     sh_stream_t stream);
 /* In-place minibatch update of every center with its members among the n rows (unit
  * weights): c = (c * w + sum of members in row order) * (1 / (w + count)), w += count,
  * unchanged when count is 0; counts[j] receives the member count.  Same roundings as

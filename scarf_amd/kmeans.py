@@ -85,6 +85,23 @@ class MiniBatchKMeans:
         self._C = None
         self._centers_host = None
         self._labels_dev = None
+        self._bufs = {}
+
+    def _buf(self, name: str, numel: int, dtype, device):
+        """A per-instance device buffer of at least numel elements (reused across calls on
+        the current stream; kernels on one stream run in order)."""
+        b = self._bufs.get(name)
+        if b is None or b.numel() < numel:
+            b = self._bufs[name] = torch.empty(max(int(numel), 1), dtype=dtype, device=device)
+        return b[:numel]
+
+    def _assign(self, t, code, ld, n, d, labels, stream):
+        k = self.n_clusters
+        cn = self._buf("cn", k, torch.float64, t.device)
+        wsb = _lib.lib().sh_kmeans_assign_workspace(n, k)
+        ws = self._buf("assign_ws", wsb, torch.uint8, t.device) if wsb else None
+        _lib.call("sh_kmeans_assign", ptr(t), code, n, d, ld, ptr(self._C), k, ptr(cn),
+                  ptr(labels), ptr(ws), wsb, stream)
 
     # -- state ---------------------------------------------------------------------------
     def _rs(self) -> np.random.RandomState:
@@ -180,11 +197,9 @@ class MiniBatchKMeans:
             self._n_since_last_reassign = 0
             reassign = True
         dev = t.device
-        labels = torch.empty(n, dtype=torch.int32, device=dev)
-        cn = torch.empty(k, dtype=torch.float64, device=dev)
-        counts = torch.empty(k, dtype=torch.int32, device=dev)
-        _lib.call("sh_kmeans_assign", ptr(t), code, n, d, ld, ptr(self._C), k, ptr(cn),
-                  ptr(labels), stream)
+        labels = self._buf("labels", n, torch.int32, dev)
+        counts = self._buf("counts", k, torch.int32, dev)
+        self._assign(t, code, ld, n, d, labels, stream)
         _lib.call("sh_kmeans_update", ptr(t), code, n, d, ld, ptr(labels), ptr(self._C), k,
                   ptr(self._ws_d), ptr(counts), stream)
         ws = self._ws_d.cpu().numpy().copy()
@@ -223,9 +238,7 @@ class MiniBatchKMeans:
             raise ValueError(f"X has {d} features, but MiniBatchKMeans is expecting "
                              f"{self._n_features} features as input.")
         labels = torch.empty(n, dtype=torch.int32, device=t.device)
-        cn = torch.empty(self.n_clusters, dtype=torch.float64, device=t.device)
-        _lib.call("sh_kmeans_assign", ptr(t), code, n, d, ld, ptr(self._C), self.n_clusters,
-                  ptr(cn), ptr(labels), stream_handle())
+        self._assign(t, code, ld, n, d, labels, stream_handle())
         return labels
 
     def predict(self, X) -> np.ndarray:
