@@ -290,6 +290,22 @@ int sh_kmeans_gather(const void *X, int x_dtype, int32_t d, int64_t ldx,
                      const int64_t *src_rows, const int32_t *dst_rows, int64_t m,
                      double *centers, sh_stream_t stream);
 
+/* ---- cell graph as a sparse matrix ----------------------------------------------------
+ * The stored kNN graph (edges i64 [nnz x 2] as (row, col), weights f64 [nnz], read in
+ * groups of k entries, the first use_k of each group kept) as a canonical CSR g: columns
+ * sorted, duplicate entries summed in input order, explicit zeros kept.  symmetric != 0:
+ * t = g + g^T - g o g^T with scipy's sparse arithmetic (each operation drops the zeros it
+ * produces); upper_only then keeps col >= row.  Writes indptr i64 [n + 1] (indptr[n] is the
+ * entry count), indices i32 and data f64 (capacity >= kept edges, twice that when
+ * symmetric) and *n_bad (device u32): the number of edges with a row or column outside
+ * [0, n), which are left out.  Replaces graph_datastore.py:474-511 (_store_to_sparse) and
+ * 1022-1075 (load_graph). */
+size_t sh_graph_csr_workspace(int64_t n, int64_t nnz, int32_t k, int32_t use_k);
+int sh_graph_csr(const int64_t *edges, const double *weights, int64_t nnz, int32_t k,
+                 int32_t use_k, int64_t n, int symmetric, int upper_only, int64_t *indptr,
+                 int32_t *indices, double *data, int64_t capacity, unsigned int *n_bad,
+                 void *workspace, size_t workspace_bytes, sh_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -150,6 +150,30 @@ def make_graph(counts, feat_idx=None, *, assay: str = "RNA", cell_key: str = "I"
     return store
 
 
+def load_graph(store: Dict, symmetric: Optional[bool] = None,
+               upper_only: Optional[bool] = None, use_k: Optional[int] = None,
+               graph_loc: Optional[str] = None):
+    """``GraphDataStore.load_graph`` (``graph_datastore.py:1022-1075``): the stored graph as a
+    scipy CSR matrix (COO when ``symmetric`` and ``upper_only``), built on the GPU.
+
+    ``graph_loc`` defaults to the latest graph; the cell count and k come from the kNN
+    table next to it (``_get_graph_ncells_k``, ``graph_datastore.py:459-472``).
+    """
+    from .sparse import graph_csr
+
+    if graph_loc is None:
+        if "latest_graph" not in store:
+            raise ValueError("no graph in the store: run make_graph first")
+        graph_loc = store["latest_graph"]
+    if graph_loc + "/edges" not in store:
+        raise ValueError(f"{graph_loc} not found in the store. Run `make_graph`")
+    knn_loc = graph_loc.rsplit("/", 1)[0]
+    n_cells, k = np.asarray(store[knn_loc + "/indices"]).shape
+    return graph_csr(store[graph_loc + "/edges"], store[graph_loc + "/weights"], n_cells, k,
+                     use_k=use_k, symmetric=symmetric is True,
+                     upper_only=upper_only is True)
+
+
 def run_mapping(ann: AnnStream, target_counts, target_ids: Sequence, source_feat_ids: Sequence,
                 target_name: str, save_k: int = 3, ref_mu: bool = True, ref_sigma: bool = True,
                 target_n_counts=None, store: Optional[Dict] = None, assay: str = "RNA"):

@@ -313,3 +313,25 @@ def test_kmeans_update_is_sequential_chain():
     okm.minibatch_update(X, C, w, np.array([0, 0, 0], dtype=np.int32))
     assert C[0, 0] == ((0.1 * 3.0 + 0.2) + 0.7 + 5.0) * (1.0 / 6.0)
     assert C[1, 0] == 9.0 and w.tolist() == [6.0, 4.0]
+
+
+# ---- graph loading (graph_datastore.py:474-511, 1022-1075) ------------------------
+def test_load_graph_oracle_hand_case():
+    """Three cells, k = 2: 0->1 (0.5) and 1->0 (0.25) are mutual, 0->2 (0.8) is one-way,
+    2->1 holds an explicit zero, 1->2 (1.0) meets 2->1's zero."""
+    from oracle import graph as og
+
+    edges = np.array([[0, 1], [0, 2], [1, 0], [1, 2], [2, 1], [2, 0]], dtype=np.uint64)
+    w = np.array([0.5, 0.8, 0.25, 1.0, 0.0, 0.3])
+    g = og.load_graph(edges, w, 3, 2)
+    assert g.nnz == 6 and g[2, 1] == 0.0  # the explicit zero is kept
+    t = og.load_graph(edges, w, 3, 2, symmetric=True).toarray()
+    want = np.array([[0.0, 0.5 + 0.25 - 0.5 * 0.25, 0.8 + 0.3 - 0.8 * 0.3],
+                     [0.5 + 0.25 - 0.5 * 0.25, 0.0, 1.0],
+                     [0.8 + 0.3 - 0.8 * 0.3, 1.0, 0.0]])
+    np.testing.assert_array_equal(t, want)
+    assert (t == t.T).all()
+    u = og.load_graph(edges, w, 3, 2, symmetric=True, upper_only=True)
+    assert u.format == "coo" and (u.row <= u.col).all() and u.nnz == 3
+    g1 = og.load_graph(edges, w, 3, 2, use_k=1)
+    assert g1.nnz == 3 and g1[0, 1] == 0.5 and g1[1, 0] == 0.25 and g1[2, 1] == 0.0
