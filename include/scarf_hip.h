@@ -306,6 +306,22 @@ int sh_graph_csr(const int64_t *edges, const double *weights, int64_t nnz, int32
                  int32_t *indices, double *data, int64_t capacity, unsigned int *n_bad,
                  void *workspace, size_t workspace_bytes, sh_stream_t stream);
 
+/* ---- CORAL alignment -------------------------------------------------------------------
+ * C = (G - n m m^T) / (n - 1) + ridge I with m = colsum / n, +inf entries set to 0: the
+ * covariance (ddof 1) of n rows from their Gram G (FP64, leading dimension ldg) and column
+ * sums, as _cov_diaged (mapping_utils.py:15-19) with ridge 1. */
+int sh_cov_from_gram(const double *G, int64_t ldg, const double *colsum, int64_t n, int32_t d,
+                     double ridge, double *C, sh_stream_t stream);
+/* C^{1/2} and C^{-1/2} of a symmetric positive definite C (FP64 [d x d]) by the coupled
+ * Newton-Schulz iteration on FP64 MFMA GEMMs, replacing scipy's fractional_matrix_power
+ * (mapping_utils.py:22-38).  Stops when ||I - Z Y||_F <= tol sqrt(d) or when the residual
+ * stops falling below 1e-6, after at most max_iter iterations; *iters and *resid report the
+ * count and the last residual.  Reads the residual back once per iteration (synchronous). */
+size_t sh_spd_sqrt_workspace(int32_t d);
+int sh_spd_sqrt(const double *C, int32_t d, double *sqrt_out, double *inv_sqrt_out,
+                int32_t max_iter, double tol, int32_t *iters, double *resid, void *workspace,
+                size_t workspace_bytes, sh_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -70,6 +70,9 @@ _SIGS = {
     "sh_kmeans_assign": (_int, [_p, _int, _i64, _i32, _i64, _p, _i32, _p, _p, _p, _sz, _p]),
     "sh_kmeans_update": (_int, [_p, _int, _i64, _i32, _i64, _p, _p, _i32, _p, _p, _p]),
     "sh_kmeans_gather": (_int, [_p, _int, _i32, _i64, _p, _p, _i64, _p, _p]),
+    "sh_cov_from_gram": (_int, [_p, _i64, _p, _i64, _i32, _f64, _p, _p]),
+    "sh_spd_sqrt_workspace": (_sz, [_i32]),
+    "sh_spd_sqrt": (_int, [_p, _i32, _p, _p, _i32, _f64, _p, _p, _p, _sz, _p]),
     "sh_graph_csr_workspace": (_sz, [_i64, _i64, _i32, _i32]),
     "sh_graph_csr": (_int, [_p, _p, _i64, _i32, _i32, _i64, _int, _int, _p, _p, _p, _i64, _p, _p, _sz, _p]),
 }

@@ -174,21 +174,19 @@ def load_graph(store: Dict, symmetric: Optional[bool] = None,
                      upper_only=upper_only is True)
 
 
-def run_mapping(ann: AnnStream, target_counts, target_ids: Sequence, source_feat_ids: Sequence,
-                target_name: str, save_k: int = 3, ref_mu: bool = True, ref_sigma: bool = True,
-                target_n_counts=None, store: Optional[Dict] = None, assay: str = "RNA"):
-    """``MappingDatastore.run_mapping`` (``mapping_datastore.py:31-209``), CORAL off.
+def project_target(ann: AnnStream, target_counts, target_ids: Sequence,
+                   source_feat_ids: Sequence, ref_mu: bool = True, ref_sigma: bool = True,
+                   target_n_counts=None, run_coral: bool = False):
+    """The target cells' embedding (FP32 device [n_target, dims]) as ``run_mapping`` reduces
+    them (``mapping_datastore.py:100-208``).
 
     The target uses the reference's size factor and subset parameters
-    (``mapping_datastore.py:106-111``, ``mapping_utils.py:188-194``); features are
-    aligned to the reference order with missing ones set to 1.0
-    (``mapping_utils.py:197-213``); neighbours are the k = save_k nearest reference
-    cells with no self repair (``mapping_datastore.py:198-208``).
+    (``mapping_datastore.py:106-111``, ``mapping_utils.py:188-194``); features are aligned
+    to the reference order with missing ones set to 1.0 (``mapping_utils.py:197-213``).
+    ``run_coral`` applies CORAL first (``mapping_datastore.py:160-175``), in which case the
+    reference's own mu/sigma are used whatever ``ref_mu`` / ``ref_sigma`` say, as in scarf.
     """
     require_gpu()
-    store = {} if store is None else store
-    if save_k > ann.k:
-        save_k = ann.k
     tcsr = _device_csr(target_counts)
     positions, missing = align_col_map(list(source_feat_ids), list(target_ids))
     if len(positions) != tcsr.n_cols:
@@ -203,6 +201,10 @@ def run_mapping(ann: AnnStream, target_counts, target_ids: Sequence, source_feat
             raise ValueError("renormalize_subset=False needs the target's per-cell totals")
         nct = torch.as_tensor(np.asarray(target_n_counts, dtype=np.float64)).cuda()
         libsize = nct
+    if run_coral:
+        from .coral import coral_projector
+        proj, _ = coral_projector(ann, tcsr, cm, missing, norm, n_counts=nct)
+        return proj.project(tcsr, norm, libsize=libsize, col_map=cm)
     mu, sigma = ann.mu, ann.sigma
     if not (ref_mu and ref_sigma):
         tst = feature_stats(tcsr, 1000, norm, col_map=cm, d=d, n_counts=nct)
@@ -217,7 +219,25 @@ def run_mapping(ann: AnnStream, target_counts, target_ids: Sequence, source_feat
             sigma = ts
     proj = Projector.build(ann.loadings, mu, sigma, clip=norm.clip,
                            disable_scaling=ann._disable_scaling, fill_missing=missing)
-    E = proj.project(tcsr, norm, libsize=libsize, col_map=cm)
+    return proj.project(tcsr, norm, libsize=libsize, col_map=cm)
+
+
+def run_mapping(ann: AnnStream, target_counts, target_ids: Sequence, source_feat_ids: Sequence,
+                target_name: str, save_k: int = 3, ref_mu: bool = True, ref_sigma: bool = True,
+                target_n_counts=None, store: Optional[Dict] = None, assay: str = "RNA",
+                run_coral: bool = False):
+    """``MappingDatastore.run_mapping`` (``mapping_datastore.py:31-209``).
+
+    Projects the target (``project_target``), then stores its k = save_k nearest reference
+    cells with no self repair (``mapping_datastore.py:198-208``).
+    """
+    require_gpu()
+    store = {} if store is None else store
+    if save_k > ann.k:
+        save_k = ann.k
+    E = project_target(ann, target_counts, target_ids, source_feat_ids, ref_mu=ref_mu,
+                       ref_sigma=ref_sigma, target_n_counts=target_n_counts,
+                       run_coral=run_coral)
     idx, dist, _ = ann.annIdx.knn_query_tensor(E, save_k)
     loc = f"{assay}/projections/{target_name}"
     store[loc + "/indices"] = idx.cpu().numpy().astype(np.uint64)

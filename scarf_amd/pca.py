@@ -221,12 +221,15 @@ def _z_params(stats: FeatureStats, disable_scaling: bool):
 
 def gram_of_rows(csr: DeviceCSR, stats: FeatureStats, params: NormParams, rows,
                  col_map=None, disable_scaling=False, sub_rows: int = 32768,
-                 precision: str = "fp64", stream=None, to_host: bool = True):
+                 precision: str = "fp64", stream=None, to_host: bool = True,
+                 center=None, scale=None):
     """(Z^T Z, column sums, n) of z over ``rows`` (device int64 tensor) in FP64: host
     arrays, or with ``to_host=False`` the Gram as a device tensor (column sums on host).
 
     ``precision="fp64"`` densifies z in FP64 and uses FP64 MFMA (parity with the
     reference's FP64 PCA on every component); ``"fp32"`` rounds z to FP32 first.
+    z = (x - center) * scale, by default the stats' mu and 1 / sigma; ``center`` /
+    ``scale`` (device FP64 [d]) replace them.
     """
     require_gpu()
     d = stats.d
@@ -238,7 +241,9 @@ def gram_of_rows(csr: DeviceCSR, stats: FeatureStats, params: NormParams, rows,
         raise ValueError(f"unknown Gram precision {precision!r}")
     G = torch.zeros((d, d), dtype=torch.float64, device=dev)
     cs = torch.zeros(d, dtype=torch.float64, device=dev)
-    center, scale = _z_params(stats, disable_scaling)
+    c0, s0 = _z_params(stats, disable_scaling)
+    center = c0 if center is None else center
+    scale = s0 if scale is None else scale
     st = stream_handle(stream)
     zb = min(n, sub_rows)
     Z = torch.empty((max(zb, 1), ld), dtype=torch.float64 if f64 else torch.float32, device=dev)
@@ -331,9 +336,11 @@ class Projector:
     @classmethod
     def build(cls, loadings: np.ndarray, mu: np.ndarray, sigma: np.ndarray,
               clip: Optional[float] = None, disable_scaling: bool = False,
-              fill_missing: Optional[np.ndarray] = None, device="cuda") -> "Projector":
+              fill_missing: Optional[np.ndarray] = None, offset: Optional[np.ndarray] = None,
+              device="cuda") -> "Projector":
         """``fill_missing`` (bool [d]) marks features absent from the input, which scarf's
-        align_features fills with 1.0 (``mapping_utils.py:208``)."""
+        align_features fills with 1.0 (``mapping_utils.py:208``); ``offset`` [D] is added
+        to every projected row."""
         L = np.asarray(loadings, dtype=np.float64)
         d, D = L.shape
         mu = np.asarray(mu, dtype=np.float64)
@@ -352,6 +359,8 @@ class Projector:
                 z1 = np.clip(z1, -clip, clip)
             zc[fm] = z1[fm]
         c = zc @ L
+        if offset is not None:
+            c = c + np.asarray(offset, dtype=np.float64)
         # one host buffer, one copy: [W (d x D) | mu (d) | 1/sigma (d) | c (D)] in FP32
         host = np.concatenate([L.astype(np.float32).ravel(), mu.astype(np.float32),
                                (1.0 / sigma).astype(np.float32), c.astype(np.float32)])

@@ -335,3 +335,28 @@ def test_load_graph_oracle_hand_case():
     assert u.format == "coo" and (u.row <= u.col).all() and u.nnz == 3
     g1 = og.load_graph(edges, w, 3, 2, use_k=1)
     assert g1.nnz == 3 and g1[0, 1] == 0.5 and g1[1, 0] == 0.25 and g1[2, 1] == 0.0
+
+
+# ---- CORAL (mapping_utils.py:15-95) ----------------------------------------------
+def test_coral_oracle_pins():
+    from oracle import coral as oc
+
+    rng = np.random.default_rng(3)
+    x = rng.normal(size=(400, 12)) @ rng.normal(size=(12, 12))
+    y = rng.normal(size=(300, 12)) * np.linspace(0.5, 3, 12) + 2
+    c = oc.cov_diaged(x)  # ddof 1, plus the identity
+    m = x.mean(0)
+    np.testing.assert_allclose(c, (x - m).T @ (x - m) / 399 + np.eye(12), rtol=1e-12,
+                               atol=1e-12)
+    s = (x - x.mean(0)) / x.std(0)
+    t = (y - y.mean(0)) / y.std(0)
+    A = oc.alignment(s, t)
+    Cs, Ct = oc.cov_diaged(s), oc.cov_diaged(t)
+    np.testing.assert_allclose(A.T @ Cs @ A, Ct, rtol=1e-9, atol=1e-9)  # re-colouring
+    w, V = np.linalg.eigh(Cs)
+    wt, Vt = np.linalg.eigh(Ct)
+    np.testing.assert_allclose(A, (V * w ** -0.5) @ V.T @ (Vt * wt ** 0.5) @ Vt.T,
+                               rtol=1e-9, atol=1e-10)
+    data, A2 = oc.coral(x, x)  # same distribution: nothing to correct
+    np.testing.assert_allclose(A2, np.eye(12), atol=1e-10)
+    np.testing.assert_allclose(data, (x - x.mean(0)) / x.std(0), atol=1e-9)
