@@ -26,7 +26,7 @@ METRIC = "cells/s norm+PCA+kNN @1M cells, recall@11 vs FP64 oracle; mapping p50/
 PEAK = {"bf16": 2.5e15, "fp32": 157.3e12, "fp8": 5.0e15, "fp64": 78.6e12}
 # HBM bytes per screen launch from FETCH_SIZE / WRITE_SIZE counter passes scaled by a 1 GiB
 # copy (profiles/screen_bf16_1m_traffic.md), keyed by (workload, screen, pool)
-SCREEN_TRAFFIC = {("1m", "bf16", 64): 4.75e11}
+SCREEN_TRAFFIC = {("1m", "bf16", 64): 4.76e11}
 
 WORKLOADS = {
     # cells, genes, groups, density, chunk, dims, k, screen, pool, pca mode, recall queries
@@ -127,6 +127,58 @@ def serving_bench(seconds: float, rate: float = 20.0, n_ref: int = 500_000, d: i
     return out
 
 
+def aux_timings(res, csr, params):
+    """Timings of the paths beside the metric, on this workload's data (not part of
+    ``value``): seed partitions with scarf's defaults (1000 centroids, batch 1000, seed
+    4466), load_graph(symmetric=True) from the host edge table, and the CORAL pieces
+    (reference covariance from the FP64 Gram, the two Newton-Schulz square roots)."""
+    import torch
+
+    from scarf_amd.coral import covariance_plus_identity, spd_sqrt
+    from scarf_amd.kmeans import MiniBatchKMeans
+    from scarf_amd.sparse import graph_csr
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = fn()
+        torch.cuda.synchronize()
+        return r, 1e3 * (time.perf_counter() - t0)
+
+    E = res.embedding
+    n, k = res.indices.shape
+    out = {}
+    km = MiniBatchKMeans(n_clusters=1000, random_state=4466, batch_size=1000, n_init=3,
+                         compute_labels=False)
+    _, out["kmeans_first_partial_fit_ms"] = timed(lambda: km.partial_fit(E[:1000]))
+
+    def rest():
+        for s in range(1000, n, 1000):
+            km.partial_fit(E[s:s + 1000])
+    _, out["kmeans_other_partial_fits_ms"] = timed(rest)
+    _, out["kmeans_predict_ms"] = timed(lambda: km.predict_tensor(E))
+    log(f"seed partitions: {out}")
+    edges = res.edges()
+    w = res.weights.cpu().numpy().ravel()
+    g, out["load_graph_symmetric_first_ms"] = timed(
+        lambda: graph_csr(edges, w, n, k, symmetric=True))
+    del g
+    g, out["load_graph_symmetric_ms"] = timed(lambda: graph_csr(edges, w, n, k, symmetric=True))
+    out["load_graph_nnz"] = int(g.nnz)
+    log(f"load_graph: {out['load_graph_symmetric_ms']:.1f} ms, nnz {g.nnz}")
+    del g
+    st = res.stats
+    T, out["coral_reference_cov_ms"] = timed(lambda: covariance_plus_identity(
+        csr, st, params.norm(), st.mu.contiguous(), (1.0 / st.sigma).contiguous()))
+    (_, _, it1, _), out["coral_sqrt_ms"] = timed(lambda: spd_sqrt(T, sqrt=True, inv_sqrt=False))
+    (_, _, it2, _), out["coral_inv_sqrt_ms"] = timed(lambda: spd_sqrt(T, sqrt=False,
+                                                                      inv_sqrt=True))
+    out["coral_newton_schulz_iters"] = [it1, it2]
+    out["coral_d"] = int(T.shape[0])
+    log(f"coral: {out}")
+    return out
+
+
 def recall_vs_exact(res, k, n, n_queries, seed=7):
     """Recall@k (and @11) and k-th distance error of the graph's self-kNN against the
     exact FP64 top-k (certified FP32 screen + FP64 re-rank, bit-equal to brute force)."""
@@ -174,6 +226,8 @@ def main():
                     help="after the timed steps, copy 1 GiB device to device once (a kernel of "
                          "known traffic that scales hardware counter readings)")
     ap.add_argument("--serving-seconds", type=float, default=12.0)
+    ap.add_argument("--aux", action="store_true",
+                    help="also time seed partitions, load_graph and CORAL (key 'aux')")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -294,6 +348,9 @@ def main():
         "mapping_p50_ms": None,
         "mapping_p99_ms": None,
     }
+
+    if args.aux:
+        out["aux"] = aux_timings(res, csr, params)
 
     if args.compare_screen:
         alt = gknn.pack_refs(res.embedding, args.compare_screen)
