@@ -43,7 +43,9 @@ def graph_csr(edges, weights, n_cells: int, k: int, use_k=None, symmetric: bool 
     nnz = e.shape[0]
     if w.shape[0] != nnz or nnz % k:
         raise ValueError("edges and weights must hold n_cells * k entries")
-    e64 = np.ascontiguousarray(e.astype(np.int64, copy=False))
+    # uint64 (scarf's "u8" edges) and int64 share bits for cell indices: view, do not copy
+    e64 = np.ascontiguousarray(e.view(np.int64) if e.dtype == np.uint64 else
+                               e.astype(np.int64, copy=False))
     dev = torch.device("cuda")
     e_d = torch.from_numpy(e64).to(dev)
     w_d = torch.from_numpy(np.ascontiguousarray(w)).to(dev)
@@ -64,8 +66,9 @@ def graph_csr(edges, weights, n_cells: int, k: int, use_k=None, symmetric: bool 
         raise ValueError(f"{int(bad.item())} edges name a cell outside [0, {n})")
     total = int(ip[-1])
     idx_t = np.int32 if max(n, total) < 2 ** 31 else np.int64
-    g = csr_matrix((data[:total].cpu().numpy(), indices[:total].cpu().numpy().astype(idx_t),
-                    ip.astype(idx_t)), shape=(n, n))
+    g = csr_matrix((data[:total].cpu().numpy(),
+                    indices[:total].cpu().numpy().astype(idx_t, copy=False),
+                    ip.astype(idx_t)), shape=(n, n), copy=False)
     g.has_sorted_indices = True
     if symmetric and upper_only:
         return coo_matrix(g)
