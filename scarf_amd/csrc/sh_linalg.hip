@@ -120,85 +120,102 @@ __global__ void k_merge_matrix(const double *__restrict__ G, int32_t d, double n
 }
 
 // ---- block Lanczos pieces -------------------------------------------------------------
-// W[q][i] = sum_j M[i][j] V[q][j]: one wave per kRowsPerWave rows of M (contiguous, read
-// once), lanes stride over j; the V values a lane loads serve all of the wave's rows.
-constexpr int kRowsPerWave = 4;
-__global__ void __launch_bounds__(256) k_symm_rows(const double *__restrict__ M, int32_t d,
-                                                   const double *__restrict__ V,
-                                                   double *__restrict__ W) {
-  const int lane = threadIdx.x & 63;
-  const int64_t i0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * kRowsPerWave;
-  if (i0 >= d) return;
-  double acc[kRowsPerWave][kLP];
+// W[q][i] = sum_j M[i][j] V[q][j]: one wave per row of M (read once, coalesced), 16 rows
+// per workgroup.  The block V (8 x d) passes through LDS 256 columns at a time, so each
+// workgroup reads it once instead of once per row; 16 waves with 4 row loads each in
+// flight stream M.
+constexpr int kSymWaves = 16, kSymCols = 256;
+__global__ void __launch_bounds__(kSymWaves * 64) k_symm_rows(const double *__restrict__ M,
+                                                              int32_t d,
+                                                              const double *__restrict__ V,
+                                                              double *__restrict__ W) {
+  __shared__ double sV[kLP][kSymCols];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int64_t i = (int64_t)blockIdx.x * kSymWaves + w;
+  const bool row_ok = i < d;
+  const double *mrow = M + (row_ok ? i : 0) * d;
+  double acc[kLP];
 #pragma unroll
-  for (int r = 0; r < kRowsPerWave; ++r)
-#pragma unroll
-    for (int q = 0; q < kLP; ++q) acc[r][q] = 0.0;
-  for (int j = lane; j < d; j += 64) {
-    double v[kLP];
-#pragma unroll
-    for (int q = 0; q < kLP; ++q) v[q] = V[(int64_t)q * d + j];
-#pragma unroll
-    for (int r = 0; r < kRowsPerWave; ++r) {
-      const double mv = (i0 + r < d) ? M[(i0 + r) * d + j] : 0.0;
-#pragma unroll
-      for (int q = 0; q < kLP; ++q) acc[r][q] += mv * v[q];
+  for (int q = 0; q < kLP; ++q) acc[q] = 0.0;
+  for (int c0 = 0; c0 < d; c0 += kSymCols) {
+    __syncthreads();
+    for (int e = threadIdx.x; e < kLP * kSymCols; e += kSymWaves * 64) {
+      const int q = e / kSymCols, c = e % kSymCols;
+      sV[q][c] = c0 + c < d ? V[(int64_t)q * d + c0 + c] : 0.0;
     }
+    __syncthreads();
+    double m[kSymCols / 64];
+#pragma unroll
+    for (int t = 0; t < kSymCols / 64; ++t) {
+      const int c = c0 + lane + 64 * t;
+      m[t] = row_ok && c < d ? mrow[c] : 0.0;
+    }
+#pragma unroll
+    for (int t = 0; t < kSymCols / 64; ++t)
+#pragma unroll
+      for (int q = 0; q < kLP; ++q) acc[q] += m[t] * sV[q][lane + 64 * t];
   }
 #pragma unroll
-  for (int r = 0; r < kRowsPerWave; ++r)
-#pragma unroll
-    for (int q = 0; q < kLP; ++q) {
-      const double s = wave_sum_f64(acc[r][q]);
-      if (lane == 0 && i0 + r < d) W[(int64_t)q * d + i0 + r] = s;
-    }
+  for (int q = 0; q < kLP; ++q) {
+    const double sw = wave_sum_f64(acc[q]);
+    if (lane == 0 && row_ok) W[(int64_t)q * d + i] = sw;
+  }
 }
 
-// H[c][q] = V[c] . W[q] for c < k (this pass's coefficients; one wave per basis vector).
+// H[c][q] = V[c] . W[q] for c < k (this pass's coefficients; one workgroup per basis
+// vector, its 4 waves over interleaved quarters of the features, summed in a fixed order).
 // Rows c of block jb (the newest) are also added into T_blk (the diagonal block of the
 // Lanczos matrix, summed over both passes; accumulate = 0 on the first pass).
 __global__ void __launch_bounds__(256) k_cgs_proj(const double *__restrict__ V, int32_t k,
                                                   int32_t d, const double *__restrict__ W,
                                                   double *__restrict__ H, int32_t jb,
                                                   int accumulate, double *__restrict__ T_blk) {
-  const int lane = threadIdx.x & 63;
-  const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (c >= k) return;
+  __shared__ double red[4][kLP];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int c = blockIdx.x;
   double acc[kLP];
 #pragma unroll
   for (int q = 0; q < kLP; ++q) acc[q] = 0.0;
   const double *vc = V + (int64_t)c * d;
-  for (int i = lane; i < d; i += 64) {
+#pragma unroll 2
+  for (int i = threadIdx.x; i < d; i += 256) {
     const double v = vc[i];
 #pragma unroll
     for (int q = 0; q < kLP; ++q) acc[q] += v * W[(int64_t)q * d + i];
   }
 #pragma unroll
   for (int q = 0; q < kLP; ++q) {
-    const double s = wave_sum_f64(acc[q]);
-    if (lane == 0) {
-      H[(int64_t)c * kLP + q] = s;
-      if (c / kLP == jb) {
-        double *t = T_blk + (c - jb * kLP) * kLP + q;
-        *t = accumulate ? *t + s : s;
-      }
+    const double sw = wave_sum_f64(acc[q]);
+    if (lane == 0) red[w][q] = sw;
+  }
+  __syncthreads();
+  if (threadIdx.x < kLP) {
+    const int q = threadIdx.x;
+    const double sum = ((red[0][q] + red[1][q]) + red[2][q]) + red[3][q];
+    H[(int64_t)c * kLP + q] = sum;
+    if (c / kLP == jb) {
+      double *t = T_blk + (c - jb * kLP) * kLP + q;
+      *t = accumulate ? *t + sum : sum;
     }
   }
 }
 
-// W[q][i] -= sum_{c < k} V[c][i] H[c][q]: a workgroup owns 64 features; its 4 waves take
-// every 4th basis vector and their partial sums are added in a fixed order (deterministic).
-__global__ void __launch_bounds__(256) k_cgs_update(const double *__restrict__ V, int32_t k,
-                                                    int32_t d, const double *__restrict__ H,
-                                                    double *__restrict__ W) {
-  __shared__ double part[4][kLP][64];
+// W[q][i] -= sum_{c < k} V[c][i] H[c][q]: a workgroup owns 64 features; its 16 waves take
+// every 16th basis vector and their partial sums are added in a fixed order (deterministic).
+constexpr int kUpdWaves = 16;
+__global__ void __launch_bounds__(kUpdWaves * 64) k_cgs_update(const double *__restrict__ V,
+                                                               int32_t k, int32_t d,
+                                                               const double *__restrict__ H,
+                                                               double *__restrict__ W) {
+  __shared__ double part[kUpdWaves][kLP][64];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int i = blockIdx.x * 64 + lane;
   double acc[kLP];
 #pragma unroll
   for (int q = 0; q < kLP; ++q) acc[q] = 0.0;
   if (i < d) {
-    for (int c = w; c < k; c += 4) {
+#pragma unroll 2
+    for (int c = w; c < k; c += kUpdWaves) {
       const double v = V[(int64_t)c * d + i];
 #pragma unroll
       for (int q = 0; q < kLP; ++q) acc[q] += v * H[(int64_t)c * kLP + q];
@@ -207,13 +224,15 @@ __global__ void __launch_bounds__(256) k_cgs_update(const double *__restrict__ V
 #pragma unroll
   for (int q = 0; q < kLP; ++q) part[w][q][lane] = acc[q];
   __syncthreads();
-  // 256 threads finish 8 x 64 outputs: thread t -> q = t / 32 (two per thread)
-  for (int e = threadIdx.x; e < kLP * 64; e += 256) {
-    const int q = e >> 6, l = e & 63;
+  // threads 0..511 finish the 8 x 64 outputs
+  if (threadIdx.x < kLP * 64) {
+    const int q = threadIdx.x >> 6, l = threadIdx.x & 63;
     const int ii = blockIdx.x * 64 + l;
     if (ii < d) {
-      const double s = ((part[0][q][l] + part[1][q][l]) + part[2][q][l]) + part[3][q][l];
-      W[(int64_t)q * d + ii] -= s;
+      double sum = part[0][q][l];
+#pragma unroll
+      for (int ww = 1; ww < kUpdWaves; ++ww) sum += part[ww][q][l];
+      W[(int64_t)q * d + ii] -= sum;
     }
   }
 }
@@ -233,28 +252,162 @@ __device__ __forceinline__ bool chol8_inplace(double (&A)[kLP][kLP]) {
     for (int t = 0; t < r; ++t) dg -= A[t][r] * A[t][r];
     ok = ok && dg > 1e-14 * smax;
     const double p = sqrt(fmax(dg, 1e-300));
+    const double ip = 1.0 / p;
     A[r][r] = p;
 #pragma unroll
     for (int c = r + 1; c < kLP; ++c) {
       double v = A[r][c];
 #pragma unroll
       for (int t = 0; t < r; ++t) v -= A[t][r] * A[t][c];
-      A[r][c] = v / p;
+      A[r][c] = v * ip;
     }
   }
   return ok;
 }
 
-// CholeskyQR2 of W [8][d] in one workgroup: Vout R = W with Vout orthonormal rows.
-// Each thread owns rows i = tid, tid + 256, ... and only touches those, so the only
-// cross-thread data are the 8 x 8 Gram and factor (in LDS).
-__global__ void __launch_bounds__(256) k_cholqr2(const double *__restrict__ W, int32_t d,
-                                                 double *__restrict__ Vout,
-                                                 double *__restrict__ R_out,
-                                                 int32_t *__restrict__ status) {
+// One CholeskyQR factorisation from the per-wave partial Grams in red (all threads call
+// it): the Gram summed over waves in a fixed order (36 threads), its Cholesky factor R
+// (thread 0) into Rc, and R^{-1} into Ri (8 threads, one column each).  ok_s is 0 when a
+// pivot is not safely positive.
+template <int NW>
+__device__ void qr_factor(const double (*red)[kLP * (kLP + 1) / 2], double *gsum,
+                          double (*Rc)[kLP], double (*Ri)[kLP], int *ok_s) {
   constexpr int NE = kLP * (kLP + 1) / 2;
-  __shared__ double red[4][NE];
-  __shared__ double Ri_s[kLP][kLP], Rt_s[kLP][kLP];
+  const int tid = threadIdx.x;
+  if (tid < NE) {
+    double v = red[0][tid];
+#pragma unroll
+    for (int ww = 1; ww < NW; ++ww) v += red[ww][tid];
+    gsum[tid] = v;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double A[kLP][kLP];
+    int e = 0;
+#pragma unroll
+    for (int a = 0; a < kLP; ++a)
+#pragma unroll
+      for (int b = a; b < kLP; ++b, ++e) {
+        A[a][b] = gsum[e];
+        A[b][a] = gsum[e];
+      }
+    *ok_s = chol8_inplace(A) ? 1 : 0;
+#pragma unroll
+    for (int a = 0; a < kLP; ++a)
+#pragma unroll
+      for (int b = 0; b < kLP; ++b) Rc[a][b] = b >= a ? A[a][b] : 0.0;
+  }
+  __syncthreads();
+  if (tid < kLP) {  // column c of R^{-1} by back substitution
+    const int c = tid;
+    double col[kLP];
+#pragma unroll
+    for (int r = 0; r < kLP; ++r) col[r] = 0.0;
+    col[c] = 1.0 / Rc[c][c];
+#pragma unroll
+    for (int r = kLP - 2; r >= 0; --r) {
+      if (r < c) {
+        double v = 0.0;
+#pragma unroll
+        for (int t = 1; t < kLP; ++t)
+          if (t > r && t <= c) v += Rc[r][t] * col[t];
+        col[r] = -v / Rc[r][r];
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < kLP; ++r) Ri[r][c] = col[r];
+  }
+  __syncthreads();
+}
+
+// CholeskyQR2 of W [8][d] in one workgroup: Vout R = W with Vout orthonormal rows and
+// R = R2 R1.  Thread t owns rows t, t + 512, ... (at most kQrRows of them, d <= 2048):
+// they are loaded once, kept in registers through both passes and stored once; the only
+// cross-thread data are the 8 x 8 Gram (per-wave partials in LDS) and the factors (LDS).
+constexpr int kQrThreads = 512, kQrWaves = kQrThreads / 64, kQrRows = 4;
+constexpr int kQrMaxD = kQrThreads * kQrRows;
+__global__ void __launch_bounds__(kQrThreads) k_cholqr2(const double *__restrict__ W, int32_t d,
+                                                        double *__restrict__ Vout,
+                                                        double *__restrict__ R_out,
+                                                        int32_t *__restrict__ status) {
+  constexpr int NE = kLP * (kLP + 1) / 2;
+  __shared__ double red[kQrWaves][NE];
+  __shared__ double gsum[NE];
+  __shared__ double Rc_s[2][kLP][kLP], Ri_s[kLP][kLP];
+  __shared__ int ok_s;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  double x[kQrRows][kLP];
+#pragma unroll
+  for (int r = 0; r < kQrRows; ++r) {
+    const int i = tid + r * kQrThreads;
+#pragma unroll
+    for (int q = 0; q < kLP; ++q) x[r][q] = i < d ? W[(int64_t)q * d + i] : 0.0;
+  }
+  for (int pass = 0; pass < 2; ++pass) {
+    double acc[NE];
+#pragma unroll
+    for (int e = 0; e < NE; ++e) acc[e] = 0.0;
+#pragma unroll
+    for (int r = 0; r < kQrRows; ++r) {
+      int e = 0;
+#pragma unroll
+      for (int a = 0; a < kLP; ++a)
+#pragma unroll
+        for (int b = a; b < kLP; ++b) acc[e++] += x[r][a] * x[r][b];
+    }
+#pragma unroll
+    for (int e = 0; e < NE; ++e) {
+      const double sw = wave_sum_f64(acc[e]);
+      if (lane == 0) red[w][e] = sw;
+    }
+    __syncthreads();
+    qr_factor<kQrWaves>(red, gsum, Rc_s[pass], Ri_s, &ok_s);
+    if (!ok_s) {
+      if (tid == 0) *status = 1;
+      return;
+    }
+    // x <- x R^{-1} (rows past d are zero and stay zero)
+#pragma unroll
+    for (int r = 0; r < kQrRows; ++r) {
+      double y[kLP];
+#pragma unroll
+      for (int q = 0; q < kLP; ++q) {
+        double v = 0.0;
+#pragma unroll
+        for (int t = 0; t <= q; ++t) v += x[r][t] * Ri_s[t][q];
+        y[q] = v;
+      }
+#pragma unroll
+      for (int q = 0; q < kLP; ++q) x[r][q] = y[q];
+    }
+    __syncthreads();  // Ri_s and red are rewritten by the next pass
+  }
+#pragma unroll
+  for (int r = 0; r < kQrRows; ++r) {
+    const int i = tid + r * kQrThreads;
+    if (i < d) {
+#pragma unroll
+      for (int q = 0; q < kLP; ++q) Vout[(int64_t)q * d + i] = x[r][q];
+    }
+  }
+  if (tid < kLP * kLP) {  // R = R2 R1 (upper triangular)
+    const int a = tid / kLP, b = tid % kLP;
+    double v = 0.0;
+    for (int t = a; t <= b; ++t) v += Rc_s[1][a][t] * Rc_s[0][t][b];
+    R_out[tid] = v;
+  }
+}
+
+// The same for d > 2048: rows streamed from global memory in both passes.
+__global__ void __launch_bounds__(kQrThreads) k_cholqr2_wide(const double *__restrict__ W,
+                                                             int32_t d,
+                                                             double *__restrict__ Vout,
+                                                             double *__restrict__ R_out,
+                                                             int32_t *__restrict__ status) {
+  constexpr int NE = kLP * (kLP + 1) / 2;
+  __shared__ double red[kQrWaves][NE];
+  __shared__ double gsum[NE];
+  __shared__ double Rc_s[2][kLP][kLP], Ri_s[kLP][kLP];
   __shared__ int ok_s;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   for (int pass = 0; pass < 2; ++pass) {
@@ -262,7 +415,7 @@ __global__ void __launch_bounds__(256) k_cholqr2(const double *__restrict__ W, i
     double acc[NE];
 #pragma unroll
     for (int e = 0; e < NE; ++e) acc[e] = 0.0;
-    for (int i = tid; i < d; i += 256) {
+    for (int i = tid; i < d; i += kQrThreads) {
       double x[kLP];
 #pragma unroll
       for (int q = 0; q < kLP; ++q) x[q] = src[(int64_t)q * d + i];
@@ -274,72 +427,16 @@ __global__ void __launch_bounds__(256) k_cholqr2(const double *__restrict__ W, i
     }
 #pragma unroll
     for (int e = 0; e < NE; ++e) {
-      const double s = wave_sum_f64(acc[e]);
-      if (lane == 0) red[w][e] = s;
+      const double sw = wave_sum_f64(acc[e]);
+      if (lane == 0) red[w][e] = sw;
     }
     __syncthreads();
-    if (tid == 0) {
-      double A[kLP][kLP];
-      int e = 0;
-#pragma unroll
-      for (int a = 0; a < kLP; ++a)
-#pragma unroll
-        for (int b = a; b < kLP; ++b, ++e) {
-          const double v = ((red[0][e] + red[1][e]) + red[2][e]) + red[3][e];
-          A[a][b] = v;
-          A[b][a] = v;
-        }
-      ok_s = chol8_inplace(A) ? 1 : 0;
-      // R^{-1} (upper triangular) column by column into LDS
-#pragma unroll
-      for (int c = 0; c < kLP; ++c) {
-        double col[kLP];
-#pragma unroll
-        for (int r = 0; r < kLP; ++r) col[r] = 0.0;
-        col[c] = 1.0 / A[c][c];
-#pragma unroll
-        for (int r = c - 1; r >= 0; --r) {
-          double v = 0.0;
-#pragma unroll
-          for (int t = r + 1; t <= c; ++t) v += A[r][t] * col[t];
-          col[r] = -v / A[r][r];
-        }
-#pragma unroll
-        for (int r = 0; r < kLP; ++r) Ri_s[r][c] = col[r];
-      }
-      // accumulated factor: R1 after the first pass, R2 * R1 after the second (rows of
-      // the product only read rows >= their own of R1, so the update is in place)
-#pragma unroll
-      for (int a = 0; a < kLP; ++a) {
-        double row[kLP];
-#pragma unroll
-        for (int b = 0; b < kLP; ++b) {
-          double v = 0.0;
-          if (pass == 0) {
-            v = b >= a ? A[a][b] : 0.0;
-          } else {
-#pragma unroll
-            for (int t = 0; t < kLP; ++t)
-              if (t >= a && t <= b) v += A[a][t] * Rt_s[t][b];
-          }
-          row[b] = v;
-        }
-#pragma unroll
-        for (int b = 0; b < kLP; ++b) Rt_s[a][b] = row[b];
-      }
-    }
-    __syncthreads();
+    qr_factor<kQrWaves>(red, gsum, Rc_s[pass], Ri_s, &ok_s);
     if (!ok_s) {
       if (tid == 0) *status = 1;
       return;
     }
-    double Ri[kLP][kLP];
-#pragma unroll
-    for (int a = 0; a < kLP; ++a)
-#pragma unroll
-      for (int b = 0; b < kLP; ++b) Ri[a][b] = Ri_s[a][b];
-    // rows of this thread: x <- x R^{-1}
-    for (int i = tid; i < d; i += 256) {
+    for (int i = tid; i < d; i += kQrThreads) {
       double x[kLP], y[kLP];
 #pragma unroll
       for (int q = 0; q < kLP; ++q) x[q] = src[(int64_t)q * d + i];
@@ -347,7 +444,7 @@ __global__ void __launch_bounds__(256) k_cholqr2(const double *__restrict__ W, i
       for (int q = 0; q < kLP; ++q) {
         double v = 0.0;
 #pragma unroll
-        for (int r = 0; r <= q; ++r) v += x[r] * Ri[r][q];
+        for (int t = 0; t <= q; ++t) v += x[t] * Ri_s[t][q];
         y[q] = v;
       }
 #pragma unroll
@@ -355,7 +452,12 @@ __global__ void __launch_bounds__(256) k_cholqr2(const double *__restrict__ W, i
     }
     __syncthreads();
   }
-  if (tid < kLP * kLP) R_out[tid] = Rt_s[tid / kLP][tid % kLP];
+  if (tid < kLP * kLP) {
+    const int a = tid / kLP, b = tid % kLP;
+    double v = 0.0;
+    for (int t = a; t <= b; ++t) v += Rc_s[1][a][t] * Rc_s[0][t][b];
+    R_out[tid] = v;
+  }
 }
 
 }  // namespace sh
@@ -410,17 +512,21 @@ extern "C" int sh_lanczos_steps(const double *M, int32_t d, double *V, int32_t k
   for (int32_t j = j0; j < j0 + n_steps; ++j) {
     const int32_t k = (j + 1) * kLP;  // basis vectors so far
     const double *Vj = V + (int64_t)j * kLP * d;
-    hipLaunchKernelGGL(k_symm_rows, dim3((uint32_t)cdiv(d, 4 * kRowsPerWave)), dim3(256), 0,
-                       st, M, d, Vj, W);
+    hipLaunchKernelGGL(k_symm_rows, dim3((uint32_t)cdiv(d, kSymWaves)),
+                       dim3(kSymWaves * 64), 0, st, M, d, Vj, W);
     // two passes of classical Gram-Schmidt; each subtracts only its own coefficients
     for (int pass = 0; pass < 2; ++pass) {
-      hipLaunchKernelGGL(k_cgs_proj, dim3((uint32_t)cdiv(k, 4)), dim3(256), 0, st, V, k, d, W,
-                         H, j, pass, T_diag + (int64_t)j * kLP * kLP);
-      hipLaunchKernelGGL(k_cgs_update, dim3((uint32_t)cdiv(d, 64)), dim3(256), 0, st, V, k, d,
-                         H, W);
+      hipLaunchKernelGGL(k_cgs_proj, dim3((uint32_t)k), dim3(256), 0, st, V, k, d, W, H, j,
+                         pass, T_diag + (int64_t)j * kLP * kLP);
+      hipLaunchKernelGGL(k_cgs_update, dim3((uint32_t)cdiv(d, 64)), dim3(kUpdWaves * 64), 0, st,
+                         V, k, d, H, W);
     }
-    hipLaunchKernelGGL(k_cholqr2, dim3(1), dim3(256), 0, st, W, d, V + (int64_t)k * d,
-                       T_sub + (int64_t)j * kLP * kLP, status);
+    if (d <= kQrMaxD)
+      hipLaunchKernelGGL(k_cholqr2, dim3(1), dim3(kQrThreads), 0, st, W, d,
+                         V + (int64_t)k * d, T_sub + (int64_t)j * kLP * kLP, status);
+    else
+      hipLaunchKernelGGL(k_cholqr2_wide, dim3(1), dim3(kQrThreads), 0, st, W, d,
+                         V + (int64_t)k * d, T_sub + (int64_t)j * kLP * kLP, status);
   }
   SH_LAUNCH_CHECK("sh_lanczos_steps");
   return SH_OK;
