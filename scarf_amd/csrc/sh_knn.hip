@@ -131,34 +131,57 @@ __global__ void k_pack_f32(const float *__restrict__ E, int64_t n, int32_t D, in
   if (aux) aux[r] = (float)s;
 }
 
-__global__ void k_pack_bf16(const float *__restrict__ E, int64_t n, int32_t D, int64_t lde,
-                            int role, int32_t K, uint16_t *__restrict__ out,
-                            float *__restrict__ aux) {
-  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= n) return;
-  const float *e = E + r * lde;
-  uint16_t *o = out + r * K;
-  double s = 0.0;
-  for (int t = 0; t < D; ++t) {
-    float x = e[t];
-    s += (double)x * (double)x;
-    o[t] = role == 0 ? f2bf(x) : f2bf(-2.0f * x);
+// BF16 rows, up to 128 per workgroup staged through LDS (at most 64 KiB): the FP32 rows are
+// read coalesced, each thread packs its own row (the squared norm summed over the row in
+// order, as the FP64 value the three-term split needs), and the packed rows are written
+// back coalesced.
+constexpr int kPackRows = 128;
+__global__ void __launch_bounds__(kPackRows) k_pack_bf16(const float *__restrict__ E, int64_t n,
+                                                        int32_t D, int64_t lde, int role,
+                                                        int32_t K, uint16_t *__restrict__ out,
+                                                        float *__restrict__ aux) {
+  extern __shared__ float pack_lds[];
+  const int RB = blockDim.x;  // rows per workgroup (<= kPackRows)
+  float *sE = pack_lds;                                 // [RB][D]
+  uint16_t *sO = (uint16_t *)(pack_lds + RB * D);       // [RB][K]
+  const int tid = threadIdx.x;
+  const int64_t r0 = (int64_t)blockIdx.x * RB;
+  const int rows = (int)min<int64_t>(RB, n - r0);
+  for (int e = tid; e < rows * D; e += RB) {
+    const int rr = e / D, t = e % D;
+    sE[e] = E[(r0 + rr) * lde + t];
   }
-  if (role == 0) {
-    // ||r||^2 as an unevaluated sum of three bf16 values (24 significant bits)
-    float hi = bf2f(f2bf((float)s));
-    double r1 = s - (double)hi;
-    float mid = bf2f(f2bf((float)r1));
-    double r2 = r1 - (double)mid;
-    float lo = bf2f(f2bf((float)r2));
-    o[D] = f2bf(hi);
-    o[D + 1] = f2bf(mid);
-    o[D + 2] = f2bf(lo);
-  } else {
-    o[D] = o[D + 1] = o[D + 2] = f2bf(1.0f);
+  __syncthreads();
+  if (tid < rows) {
+    const float *e = sE + tid * D;
+    uint16_t *o = sO + tid * K;
+    double s = 0.0;
+    for (int t = 0; t < D; ++t) {
+      float x = e[t];
+      s += (double)x * (double)x;
+      o[t] = role == 0 ? f2bf(x) : f2bf(-2.0f * x);
+    }
+    if (role == 0) {
+      // ||r||^2 as an unevaluated sum of three bf16 values (24 significant bits)
+      float hi = bf2f(f2bf((float)s));
+      double r1 = s - (double)hi;
+      float mid = bf2f(f2bf((float)r1));
+      double r2 = r1 - (double)mid;
+      float lo = bf2f(f2bf((float)r2));
+      o[D] = f2bf(hi);
+      o[D + 1] = f2bf(mid);
+      o[D + 2] = f2bf(lo);
+    } else {
+      o[D] = o[D + 1] = o[D + 2] = f2bf(1.0f);
+    }
+    for (int t = D + 3; t < K; ++t) o[t] = 0;
+    if (aux) aux[r0 + tid] = (float)s;
   }
-  for (int t = D + 3; t < K; ++t) o[t] = 0;
-  if (aux) aux[r] = (float)s;
+  __syncthreads();
+  // K is even: write 32-bit pairs
+  const uint32_t *s32 = (const uint32_t *)sO;
+  uint32_t *o32 = (uint32_t *)(out + r0 * K);
+  for (int e = tid; e < rows * K / 2; e += RB) o32[e] = s32[e];
 }
 
 // ---------------------------------------------------------------------------------
@@ -822,9 +845,14 @@ extern "C" int sh_knn_pack(const float *E, int64_t n, int32_t D, int64_t lde, in
   if (dtype == SH_F32)
     hipLaunchKernelGGL(k_pack_f32, grid, block, 0, (hipStream_t)stream, E, n, D, lde, role, K,
                        (float *)out, aux_out);
-  else if (dtype == SH_BF16)
-    hipLaunchKernelGGL(k_pack_bf16, grid, block, 0, (hipStream_t)stream, E, n, D, lde, role, K,
-                       (uint16_t *)out, aux_out);
+  else if (dtype == SH_BF16) {
+    const size_t per_row = D * sizeof(float) + K * sizeof(uint16_t);
+    int rb = kPackRows;
+    while (rb > 1 && (size_t)rb * per_row > 65536) rb /= 2;
+    SH_ARG((size_t)rb * per_row <= 65536, "sh_knn_pack: D=%d too large for the BF16 pack", D);
+    hipLaunchKernelGGL(k_pack_bf16, dim3((uint32_t)cdiv(n, rb)), dim3(rb), (size_t)rb * per_row,
+                       (hipStream_t)stream, E, n, D, lde, role, K, (uint16_t *)out, aux_out);
+  }
   else
     hipLaunchKernelGGL(k_pack_fp8, grid, block, 0, (hipStream_t)stream, E, n, D, lde, role, K,
                        (uint8_t *)out, aux_out);
