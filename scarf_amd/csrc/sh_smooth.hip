@@ -147,8 +147,22 @@ __global__ void __launch_bounds__(256) k_smooth_rows(
         minw = min(minw, float_ord(w));
     }
   }
-  if (zero) atomicOr(&chunk_zero[c], 1);
-  if (minw != 0xffffffffu) atomicMin(&chunk_minw[c], minw);
+  // one atomic per wave when all 64 rows are present and share a chunk (min and or do not
+  // depend on order); lanes past n returned above, so a partial wave takes the per-row path
+  const bool full = __ballot(1) == ~0ull;
+  const int64_t c_first = __shfl(c, 0, 64);
+  if (full && __ballot(c != c_first) == 0) {
+    const bool any_zero = __ballot(zero) != 0;
+    uint32_t m = minw;
+    for (int o = 32; o >= 1; o >>= 1) m = min(m, (uint32_t)__shfl_xor((int)m, o, 64));
+    if ((threadIdx.x & 63) == 0) {
+      if (any_zero) atomicOr(&chunk_zero[c], 1);
+      if (m != 0xffffffffu) atomicMin(&chunk_minw[c], m);
+    }
+  } else {
+    if (zero) atomicOr(&chunk_zero[c], 1);
+    if (minw != 0xffffffffu) atomicMin(&chunk_minw[c], minw);
+  }
 }
 
 __global__ void k_global_min(const uint32_t *__restrict__ chunk_minw,
