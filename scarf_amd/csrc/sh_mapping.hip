@@ -26,7 +26,18 @@ __global__ void k_ms_scatter(const int64_t *__restrict__ idx, const float *__res
   const int64_t q = t / k;
   const int g = groups ? groups[q] : 0;
   if (g < 0 || g >= n_groups) return;
-  if (t % k == 0) atomicAdd(&cnt[g], 1);
+  // queries per group: one atomic per wave when the wave's queries share a group
+  const bool head = t % k == 0;
+  const uint64_t hm = __ballot(head);
+  if (hm) {
+    const int lead = __ffsll((unsigned long long)hm) - 1;
+    const int g0 = __shfl(g, lead, 64);
+    if (__ballot(head && g != g0) == 0) {
+      if ((int)(threadIdx.x & 63) == lead) atomicAdd(&cnt[g0], (int)__popcll(hm));
+    } else if (head) {
+      atomicAdd(&cnt[g], 1);
+    }
+  }
   const int64_t r = idx[t];
   if (r < 0 || r >= n_ref) return;
   const double w = weighted ? 1.0 / (log1p((double)dist[t]) + 1.0) : fixed_weight;

@@ -65,9 +65,22 @@ __global__ void __launch_bounds__(64 * kStatsWaves) k_norm_stats(
   const int64_t c_end = min(n_rows, (chunk + 1) * chunk_rows);
   const int64_t r0 = chunk * chunk_rows + j_in * slab_rows;
   const int64_t r1 = min(c_end, r0 + slab_rows);
-  for (int64_t r = r0 + wave; r < r1; r += kStatsWaves) {
-    const int64_t a = rowptr[r], b = rowptr[r + 1];
-    const double l = libsize[r];
+  // row pointers and library sizes of the wave's next 64 rows come in one load per lane
+  int64_t ra = 0, rb = 0;
+  double rl = 1.0;
+  int ri = 64;
+  for (int64_t r = r0 + wave; r < r1; r += kStatsWaves, ++ri) {
+    if (ri == 64) {
+      const int64_t rr = r + (int64_t)lane * kStatsWaves;
+      if (rr < r1) {
+        ra = rowptr[rr];
+        rb = rowptr[rr + 1];
+        rl = libsize[rr];
+      }
+      ri = 0;
+    }
+    const int64_t a = __shfl(ra, ri, 64), b = __shfl(rb, ri, 64);
+    const double l = __shfl(rl, ri, 64);
     for (int64_t t = a + lane; t < b; t += 64) {
       int f = map_col(col_map, cols[t], n_in);
       if (f >= c0 && f < cend) {
