@@ -33,6 +33,7 @@ CASES = {
     "k_equals_first_block": lambda: (blobs(7, 1000, 6, 12), 100, 100, 100, None, None),
     "wide_k150_d70": lambda: (blobs(8, 3000, 70, 40, scale=3.0), 150, 500, 500, None, None),
     "outliers_cap": lambda: (outliers(21), 45, 50, 50, None, 100),
+    "two_clusters_big_init": lambda: (blobs(11, 400, 4, 2), 2, 1000, 150, 5000, None),
 }
 
 

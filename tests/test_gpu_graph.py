@@ -59,6 +59,16 @@ def test_graph_csr_matches_scipy(cuda, use_k, mode):
         assert (abs(got - got.T) > 0).nnz == 0
 
 
+def test_graph_csr_single_cell(cuda):
+    """One cell whose only neighbour is itself: a 1 x 1 matrix, symmetrised to a + a - a*a."""
+    from scarf_amd.sparse import graph_csr
+    edges = np.array([[0, 0]], dtype=np.uint64)
+    for w in (np.array([0.25]), np.array([0.0])):
+        for sym in (False, True):
+            same(graph_csr(edges, w, 1, 1, symmetric=sym),
+                 og.load_graph(edges, w, 1, 1, symmetric=sym))
+
+
 def test_graph_csr_duplicates_and_bad_edges(cuda):
     """Two copies of an edge are summed (scipy's coo -> csr); an edge naming a cell outside
     the graph is an error, not a write."""

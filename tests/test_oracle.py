@@ -262,7 +262,7 @@ def test_target_classes_oracle_hand_cases():
 # ---- MiniBatchKMeans seed partitions (ann.py:328-346) -------------------------------
 @pytest.mark.parametrize("case", ["blobs_k8", "small_blocks_k40", "init_subsample",
                                   "explicit_init_size", "k_equals_first_block",
-                                  "outliers_cap"])
+                                  "outliers_cap", "two_clusters_big_init"])
 def test_kmeans_oracle_matches_sklearn(case):
     """The restated MiniBatchKMeans reproduces scikit-learn's seeds, labels, counts and
     centers bit for bit, through the init subsample, empty-cluster reassignments and the
