@@ -173,11 +173,16 @@ int32_t sh_knn_pack_k(int dtype, int32_t D);
 int64_t sh_knn_pack_row_bytes(int dtype, int32_t D);
 int sh_knn_pack(const float *E, int64_t n, int32_t D, int64_t lde, int dtype, int role,
                 void *out, float *aux_out, sh_stream_t stream);
+/* seed_refs (NULL, or one reference row per query): each block of 256 consecutive
+ * queries first screens the block's seed rows and starts its threshold at their pool-th
+ * smallest key, which skips candidates but keeps the same pools.  The seed rows of a
+ * block must be distinct; rows outside a slice are ignored for it.  Self queries in
+ * locality order pass their own reference rows. */
 int sh_knn_screen(const void *qpack, int64_t nq, const void *rpack, int64_t nr,
                   int32_t D, int dtype, const float *q_aux, const float *r_aux,
-                  int32_t pool, int32_t n_slices, uint64_t *cand_keys,
-                  int32_t *cand_cnt, void *workspace, size_t workspace_bytes,
-                  sh_stream_t stream);
+                  int32_t pool, int32_t n_slices, const int64_t *seed_refs,
+                  uint64_t *cand_keys, int32_t *cand_cnt, void *workspace,
+                  size_t workspace_bytes, sh_stream_t stream);
 size_t sh_knn_screen_workspace(int64_t nq, int32_t pool, int32_t n_slices);
 /* Self queries: query q is reference self_idx[q] (self_idx non-NULL) or
  * self_offset + q (self_offset >= 0); k + 1 neighbours are ranked and the query's

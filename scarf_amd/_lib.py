@@ -51,7 +51,8 @@ _SIGS = {
     "sh_knn_pack_k": (_i32, [_int, _i32]),
     "sh_knn_pack_row_bytes": (_i64, [_int, _i32]),
     "sh_knn_pack": (_int, [_p, _i64, _i32, _i64, _int, _int, _p, _p, _p]),
-    "sh_knn_screen": (_int, [_p, _i64, _p, _i64, _i32, _int, _p, _p, _i32, _i32, _p, _p, _p, _sz, _p]),
+    "sh_knn_screen": (_int, [_p, _i64, _p, _i64, _i32, _int, _p, _p, _i32, _i32, _p, _p, _p, _p, _sz,
+                            _p]),
     "sh_knn_screen_workspace": (_sz, [_i64, _i32, _i32]),
     "sh_knn_rerank": (_int, [_p, _i64, _i64, _p, _i64, _i64, _i32, _p, _p, _i32, _i32, _i32, _i64, _p, _f64, _f32, _p, _p, _p, _p]),
     "sh_knn_exact": (_int, [_p, _i64, _i64, _p, _i64, _p, _i64, _i64, _i32, _i32, _i64, _p, _p, _p, _p, _p, _p]),

@@ -294,8 +294,8 @@ __device__ __forceinline__ void append_quad(const floatx16 &acc, float tau, uint
 template <class Op, int K, int CAP>
 __global__ void __launch_bounds__(256, (ScreenShape<Op, K>::MINB)) k_screen(
     const typename Op::Frag *__restrict__ Qp_, int64_t nq, const unsigned char *__restrict__ Rp,
-    int64_t nr, int64_t slice_len, int S, int pool, uint64_t *__restrict__ buf,
-    uint64_t *__restrict__ out, int32_t *__restrict__ out_cnt) {
+    int64_t nr, int64_t slice_len, int S, int pool, const int64_t *__restrict__ seeds,
+    uint64_t *__restrict__ buf, uint64_t *__restrict__ out, int32_t *__restrict__ out_cnt) {
   typedef typename Op::Frag Frag;
   constexpr int QG = ScreenShape<Op, K>::QG;
   constexpr int QPW = 32 * QG;                // queries per wave
@@ -384,6 +384,116 @@ __global__ void __launch_bounds__(256, (ScreenShape<Op, K>::MINB)) k_screen(
             (__attribute__((address_space(3))) void *)(&lds_r[(B)][1024 * j]), 16, 0, 0); \
       }                                                                                  \
     }                                                                                    \
+  }
+
+  // Seeding: before the scan, every query of the workgroup screens the workgroup's QPB
+  // seed references (seeds[qb0 .. qb0 + QPB), distinct rows of the reference set, e.g.
+  // the block's own rows when queries are the references in locality order) and starts
+  // from the pool-th smallest of those keys instead of an open threshold.  The seeds are
+  // references of the slice, screened by the same MFMA arithmetic, so at least `pool`
+  // references have a key at or below that threshold and the kept pool is the same; the
+  // scan only skips candidates that could not have stayed.  (A 2^-16 relative margin
+  // keeps this true even if a key were rounded differently in another tile position.)
+  static_assert(QPB == 256, "one seed slot per thread");
+  static_assert(QPB % RT == 0 && (QPB / 32) * 16 <= HALFC, "seed stages must fit the buffer");
+  if (seeds != nullptr && r_begin < r_end) {
+    __shared__ uint32_t seed_row[QPB];
+    __shared__ uint32_t seed_bad[QPB / 32];  // bit s of word i: slot 32 i + s has no seed
+    {
+      const int64_t q = qb0 + threadIdx.x;
+      const int64_t r = q < nq ? seeds[q] : -1;
+      const bool ok = r >= r_begin && r < r_end;
+      seed_row[threadIdx.x] = (uint32_t)(ok ? r : r_begin);
+      const uint64_t bad = __ballot(!ok);
+      if (lane == 0) {
+        seed_bad[2 * w] = (uint32_t)bad;
+        seed_bad[2 * w + 1] = (uint32_t)(bad >> 32);
+      }
+    }
+    __syncthreads();
+    for (int s0 = 0; s0 < QPB; s0 += RT) {
+      // gather the stage's seed rows: LDS row rr holds seed slot s0 + rr (rows the chunk
+      // layout touches past the stage re-read a valid row; they are never used)
+#pragma unroll
+      for (int i = 0; i < JPW; ++i) {
+        const int j = w + 4 * i;
+        if (j < NINSTR) {
+          const int slot = s0 + chrow[i];
+          const uint32_t row = seed_row[slot < QPB ? slot : 0];
+          const unsigned char *src =
+              Rp + (int64_t)row * GROWB + (choff[i] - (uint32_t)(chrow[i] * GROWB));
+          __builtin_amdgcn_global_load_lds(
+              (const void *)src, (__attribute__((address_space(3))) void *)(&lds_r[0][1024 * j]),
+              16, 0, 0);
+        }
+      }
+      __builtin_amdgcn_s_waitcnt(0);
+      __syncthreads();
+#pragma unroll
+      for (int sub = 0; sub < RT / 32; ++sub) {
+        floatx16 acc[QG];
+        if constexpr (Op::kScaled) {
+          floatx16 nrm;
+#pragma unroll
+          for (int v = 0; v < 16; ++v)
+            nrm[v] = *reinterpret_cast<const float *>(
+                &lds_r[0][(sub * 32 + 8 * (v >> 2) + 4 * h + (v & 3)) * ROWB + K * Op::ESZ]);
+#pragma unroll
+          for (int g = 0; g < QG; ++g) acc[g] = nrm;
+        } else {
+#pragma unroll
+          for (int g = 0; g < QG; ++g)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) acc[g][v] = 0.f;
+        }
+        const unsigned char *arow = &lds_r[0][(sub * 32 + n) * ROWB + h * HALF * Op::ESZ];
+        int ascale = 127;
+        if constexpr (Op::kScaled)
+          ascale = *reinterpret_cast<const int32_t *>(
+              &lds_r[0][(sub * 32 + n) * ROWB + K * Op::ESZ + 4]);
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+          Frag a = *reinterpret_cast<const Frag *>(arow + s * Op::EPL * Op::ESZ);
+#pragma unroll
+          for (int g = 0; g < QG; ++g) {
+            if constexpr (Op::kScaled)
+              acc[g] = Op::mma_s(a, bq[g][s], acc[g], ascale, qscale[g]);
+            else
+              acc[g] = Op::mma(a, bq[g][s], acc[g]);
+          }
+        }
+        const uint32_t badw = seed_bad[(s0 >> 5) + sub];
+#pragma unroll
+        for (int g = 0; g < QG; ++g) {
+#pragma unroll
+          for (int v = 0; v < 16; ++v)
+            if ((badw >> (8 * (v >> 2) + 4 * h + (v & 3))) & 1u) acc[g][v] = INFINITY;
+          const int64_t q = qb0 + qw0 + g * 32 + n;
+          uint64_t *half = buf + ((qvalid[g] ? q : 0) * S + slice) * CAP + h * HALFC;
+          append_quad<0>(acc[g], tau[g], 0u, half, cnt[g]);
+          append_quad<4>(acc[g], tau[g], 0u, half, cnt[g]);
+          append_quad<8>(acc[g], tau[g], 0u, half, cnt[g]);
+          append_quad<12>(acc[g], tau[g], 0u, half, cnt[g]);
+        }
+      }
+      __syncthreads();  // the next stage overwrites the buffer
+    }
+    // each query's threshold: the pool-th smallest seed key (open below `pool` seeds);
+    // the seed keys themselves are dropped, the scan meets those references again
+#pragma unroll
+    for (int g = 0; g < QG; ++g) {
+      for (int src = 0; src < 32; ++src) {
+        const int64_t q = qb0 + qw0 + g * 32 + src;
+        if (q >= nq) break;
+        const int c0 = __builtin_amdgcn_readlane(cnt[g], src);
+        const int c1 = __builtin_amdgcn_readlane(cnt[g], src + 32);
+        float tnew;
+        compact_pair<CAP>(buf + (q * S + slice) * CAP, c0, c1, pool, nullptr, nullptr, tnew);
+        if (n == src)
+          tau[g] = tnew < kOpen ? fminf(tnew + fabsf(tnew) * 0x1p-16f, kOpen) : kOpen;
+      }
+      cnt[g] = 0;
+    }
   }
 
   int b = 0;
@@ -869,18 +979,18 @@ extern "C" size_t sh_knn_screen_workspace(int64_t nq, int32_t pool, int32_t n_sl
 template <class Op, int K, int CAP>
 static void launch_screen(hipStream_t st, const void *qpack, int64_t nq,
                           const void *rpack, int64_t nr, int64_t slice_len, int S, int pool,
-                          uint64_t *buf, uint64_t *out, int32_t *cnt) {
+                          const int64_t *seeds, uint64_t *buf, uint64_t *out, int32_t *cnt) {
   dim3 grid((uint32_t)cdiv(nq, ScreenShape<Op, K>::QPB), S);
   hipLaunchKernelGGL((k_screen<Op, K, CAP>), grid, dim3(256), 0, st,
                      (const typename Op::Frag *)qpack, nq, (const unsigned char *)rpack, nr,
-                     slice_len, S, pool, buf, out, cnt);
+                     slice_len, S, pool, seeds, buf, out, cnt);
 }
 
 extern "C" int sh_knn_screen(const void *qpack, int64_t nq, const void *rpack, int64_t nr,
                              int32_t D, int dtype, const float *q_aux, const float *r_aux,
-                             int32_t pool, int32_t n_slices, uint64_t *cand_keys,
-                             int32_t *cand_cnt, void *workspace, size_t workspace_bytes,
-                             sh_stream_t stream) {
+                             int32_t pool, int32_t n_slices, const int64_t *seed_refs,
+                             uint64_t *cand_keys, int32_t *cand_cnt, void *workspace,
+                             size_t workspace_bytes, sh_stream_t stream) {
   (void)q_aux;
   (void)r_aux;
   SH_ARG(nq >= 0 && nr > 0 && D > 0, "sh_knn_screen: bad sizes nq=%lld nr=%lld D=%d",
@@ -907,11 +1017,11 @@ extern "C" int sh_knn_screen(const void *qpack, int64_t nq, const void *rpack, i
 #define SH_SCREEN_CASE(OP, KV)                                                               \
   if (K == KV) {                                                                             \
     if (big)                                                                                 \
-      launch_screen<OP, KV, 512>(st, qpack, nq, rpack, nr, slice_len, S, pool, buf,          \
-                                 cand_keys, cand_cnt);                                       \
+      launch_screen<OP, KV, 512>(st, qpack, nq, rpack, nr, slice_len, S, pool, seed_refs,    \
+                                 buf, cand_keys, cand_cnt);                                  \
     else                                                                                     \
-      launch_screen<OP, KV, 256>(st, qpack, nq, rpack, nr, slice_len, S, pool, buf,          \
-                                 cand_keys, cand_cnt);                                       \
+      launch_screen<OP, KV, 256>(st, qpack, nq, rpack, nr, slice_len, S, pool, seed_refs,    \
+                                 buf, cand_keys, cand_cnt);                                  \
     SH_LAUNCH_CHECK("k_screen");                                                             \
     return SH_OK;                                                                            \
   }

@@ -156,12 +156,14 @@ def permute_rows(src: "torch.Tensor", perm: "torch.Tensor", scatter: bool = Fals
 def search(Q: "torch.Tensor", R: "torch.Tensor", refs: PackedRefs, k: int, pool: int = 64,
            self_offset: Optional[int] = None, n_slices: Optional[int] = None,
            certify: Optional[bool] = None, self_idx: Optional["torch.Tensor"] = None,
-           order: Optional[bool] = None, stream=None):
+           order: Optional[bool] = None, seed: bool = True, stream=None):
     """Top-k of each query row of ``Q`` against ``R`` (both FP32 [*, D] on device).
 
     Returns (idx int64 [nq, k], dist f32 [nq, k], flags int32 [nq]).  ``order`` (default:
     batches of at least ORDER_MIN_QUERIES rows) screens the queries in locality order;
-    the results are the same either way.
+    with ``self_offset`` and ``seed``, each block of locality-ordered queries also seeds
+    the screen threshold with its own reference rows.  The results are the same either
+    way.
     """
     require_gpu()
     nq = Q.shape[0]
@@ -171,6 +173,7 @@ def search(Q: "torch.Tensor", R: "torch.Tensor", refs: PackedRefs, k: int, pool:
         return _search(Q, R, refs, k, pool, self_offset, n_slices, certify, self_idx, stream)
     perm = locality_order(Q, stream)
     Qs = permute_rows(Q, perm, stream=stream)
+    seeds = None
     if self_idx is not None:
         si = self_idx.to(device=Q.device, dtype=torch.int64).contiguous()
         if si.numel() != nq:
@@ -178,17 +181,19 @@ def search(Q: "torch.Tensor", R: "torch.Tensor", refs: PackedRefs, k: int, pool:
         si_s = permute_rows(si, perm, stream=stream).view(-1)
     elif self_offset is not None:
         si_s = perm + int(self_offset)
+        if seed:  # distinct rows of R, near each other in locality order
+            seeds = si_s
     else:
         si_s = None
     idx_s, dist_s, flags_s = _search(Qs, R, refs, k, pool, None, n_slices, certify, si_s,
-                                     stream)
+                                     stream, seeds=seeds)
     idx = permute_rows(idx_s, perm, scatter=True, stream=stream)
     dist = permute_rows(dist_s, perm, scatter=True, stream=stream)
     flags = permute_rows(flags_s.view(-1, 1), perm, scatter=True, stream=stream).view(-1)
     return idx, dist, flags
 
 
-def _search(Q, R, refs, k, pool, self_offset, n_slices, certify, self_idx, stream):
+def _search(Q, R, refs, k, pool, self_offset, n_slices, certify, self_idx, stream, seeds=None):
     nq, D = Q.shape
     nr = R.shape[0]
     dev = Q.device
@@ -214,12 +219,14 @@ def _search(Q, R, refs, k, pool, self_offset, n_slices, certify, self_idx, strea
     st = stream_handle(stream)
     qp, _, _ = pack(Q, refs.dtype, 1, stream)
     S = _fix_slices(nr, n_slices if n_slices else default_slices(nq, nr))
+    if S > 1:  # seeds pay off when every block sees all of its seed rows
+        seeds = None
     cand = torch.empty((nq, S, pool), dtype=torch.int64, device=dev)
     cnt = torch.empty((nq, S), dtype=torch.int32, device=dev)
     ws, nbytes = workspace("screen").get(_lib.lib().sh_knn_screen_workspace(nq, pool, S))
     with kernel_timer("screen", stream, flops=2.0 * nq * nr * D):
         _lib.call("sh_knn_screen", ptr(qp), nq, ptr(refs.data), nr, D, code, None, None, pool,
-                  S, ptr(cand), ptr(cnt), ptr(ws), nbytes, st)
+                  S, ptr(seeds), ptr(cand), ptr(cnt), ptr(ws), nbytes, st)
     if certify is None:
         certify = refs.dtype in ("fp32", "f32")
     err_coef = -1.0

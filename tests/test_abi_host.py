@@ -46,7 +46,7 @@ def test_pure_queries_without_gpu():
     assert rc == _lib.SH_EINVAL
     assert b"bad sizes" in L.sh_last_error()
     rc = L.sh_knn_screen(None, 10, None, 100, 50, _lib.SH_BF16, None, None, 500, 1, None, None,
-                         None, 0, None)
+                         None, None, 0, None)
     assert rc == _lib.SH_EINVAL
 
 
