@@ -226,7 +226,8 @@ constexpr float kOpen = FLT_MAX;       // threshold of a pool that is not full y
 
 // QG: 32-query groups per wave; MINB: resident workgroups per CU the kernel is compiled
 // for (its register budget).  Two groups at three workgroups per CU measured faster than
-// four groups at two (BF16 K=64, 1M x 1M: 215 ms against 231 ms).
+// four groups at two (BF16 K=64, 1M x 1M: 215 ms against 231 ms) and than one group at
+// four (176 ms against 184 ms).
 template <class Op, int K>
 struct ScreenShape {
   static constexpr int QG = 2;
@@ -394,12 +395,12 @@ __global__ void __launch_bounds__(256, (ScreenShape<Op, K>::MINB)) k_screen(
   // references have a key at or below that threshold and the kept pool is the same; the
   // scan only skips candidates that could not have stayed.  (A 2^-16 relative margin
   // keeps this true even if a key were rounded differently in another tile position.)
-  static_assert(QPB == 256, "one seed slot per thread");
+  static_assert(QPB % 64 == 0 && QPB <= 256, "at most one seed slot per thread");
   static_assert(QPB % RT == 0 && (QPB / 32) * 16 <= HALFC, "seed stages must fit the buffer");
   if (seeds != nullptr && r_begin < r_end) {
     __shared__ uint32_t seed_row[QPB];
     __shared__ uint32_t seed_bad[QPB / 32];  // bit s of word i: slot 32 i + s has no seed
-    {
+    if (threadIdx.x < QPB) {  // whole waves
       const int64_t q = qb0 + threadIdx.x;
       const int64_t r = q < nq ? seeds[q] : -1;
       const bool ok = r >= r_begin && r < r_end;
