@@ -49,13 +49,13 @@ def _assert_same_pools(a, b):
     assert torch.equal(ca[m], cb[m])
 
 
-@pytest.mark.parametrize("screen,pool", [("bf16", 64), ("bf16", 128), ("fp32", 64),
-                                         ("fp8", 64)])
-def test_seeded_screen_same_pools(cuda, screen, pool):
+@pytest.mark.parametrize("screen,pool,D", [("bf16", 64, 50), ("bf16", 128, 50), ("fp32", 64, 50),
+                                           ("fp8", 64, 50), ("fp8", 128, 100)])
+def test_seeded_screen_same_pools(cuda, screen, pool, D):
     import torch
     from scarf_amd.knn import locality_order, pack_refs, permute_rows
     n = 9000  # 35 full blocks of 256 queries and a tail of 40
-    R = _clustered(n, 50, 31, dup_rows=80)
+    R = _clustered(n, D, 31, dup_rows=80)
     Rt = torch.from_numpy(R).cuda()
     refs = pack_refs(Rt, screen)
     perm = locality_order(Rt)
