@@ -268,10 +268,12 @@ __device__ __forceinline__ bool chol8_inplace(double (&A)[kLP][kLP]) {
 // One CholeskyQR factorisation from the per-wave partial Grams in red (all threads call
 // it): the Gram summed over waves in a fixed order (36 threads), its Cholesky factor R
 // (thread 0) into Rc, and R^{-1} into Ri (8 threads, one column each).  ok_s is 0 when a
-// pivot is not safely positive.
+// pivot is not safely positive.  Inlined: as a call, the caller's live rows were saved
+// and restored through scratch memory around it.
 template <int NW>
-__device__ void qr_factor(const double (*red)[kLP * (kLP + 1) / 2], double *gsum,
-                          double (*Rc)[kLP], double (*Ri)[kLP], int *ok_s) {
+__device__ __forceinline__ void qr_factor(const double (*red)[kLP * (kLP + 1) / 2],
+                                          double *gsum, double (*Rc)[kLP], double (*Ri)[kLP],
+                                          int *ok_s) {
   constexpr int NE = kLP * (kLP + 1) / 2;
   const int tid = threadIdx.x;
   if (tid < NE) {
