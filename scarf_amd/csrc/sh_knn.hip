@@ -19,6 +19,8 @@
 // only tiles with a hit (rare once tau has settled) take the insertion path.  When
 // a buffer nears CAP the owning wave sorts it (bitonic across the wave) and keeps
 // the best `pool`.  Each wave owns its queries, so no cross-wave synchronisation.
+// Optionally each query block first screens a set of seed reference rows (for self
+// queries in locality order: the block's own rows) and starts tau at their pool-th key.
 #include <float.h>
 
 #include "sh_common.h"
