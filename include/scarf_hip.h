@@ -144,6 +144,20 @@ int sh_dgemm(int transA, int64_t m, int64_t n, int64_t k, double alpha, const do
              int64_t lda, const double *B, int64_t ldb, double beta, double *C, int64_t ldc,
              sh_stream_t stream);
 
+/* Dense blocks of normalised values (x FP32 or FP64 [n x ldx], SH_F32 / SH_F64), as
+ * scarf's callers hand them to AnnStream.  z = (x - center) / sigma with an IEEE division
+ * (center and sigma FP64 [d], both NULL: no scaling), clipped to +-clip when clip > 0.
+ * sh_dense_z writes z (FP64 [n x ldz]); sh_dense_project writes z . W (W FP64 [d x D],
+ * out FP64 [n x ldo], D <= 128), features summed in increasing order per row.
+ * Replaces: ann.py:191-192 (transform_z) and ann.py:126-138 (reducer on a numpy block),
+ *           as called at mapping_datastore.py:204, knn_utils.py:42, metrics.py:312-315. */
+int sh_dense_z(const void *x, int x_dtype, int64_t n, int32_t d, int64_t ldx,
+               const double *center, const double *sigma, double clip, double *z, int64_t ldz,
+               sh_stream_t stream);
+int sh_dense_project(const void *x, int x_dtype, int64_t n, int32_t d, int64_t ldx,
+                     const double *center, const double *sigma, double clip, const double *W,
+                     int32_t D, double *out, int64_t ldo, sh_stream_t stream);
+
 /* Sparse z-score projection: E[r] = sum_f z_rf W[f] with z = (x - mu) * inv_sigma
  * (clipped to +-clip when clip > 0), evaluated as c + sum over the row's non-zeros of
  * (z(x) - z(0)) * W[f], where c = sum_f z(0)_f W[f] (+ alignment fill, see mapping)
@@ -193,6 +207,9 @@ int sh_knn_rerank(const float *Q, int64_t nq, int64_t ldq, const float *R, int64
                   int32_t k, int64_t self_offset, const int64_t *self_idx,
                   double err_coef, float r_norm_max, int64_t *idx_out, float *dist_out,
                   int32_t *flags_out, sh_stream_t stream);
+/* FP64 brute force for the listed query rows (all rows when q_rows is NULL).  flags_out
+ * is read and rewritten: an SH_FLAG_UNCERTIFIED bit already set for a row (by the
+ * re-rank whose certificate failed) is kept, the self bits are replaced. */
 int sh_knn_exact(const float *Q, int64_t nq, int64_t ldq, const int64_t *q_rows,
                  int64_t n_sel, const float *R, int64_t nr, int64_t ldr, int32_t D,
                  int32_t k, int64_t self_offset, const int64_t *self_idx,
@@ -234,7 +251,10 @@ size_t sh_smooth_knn_workspace(int64_t n, int64_t chunk_size);
  * For each target group g: ms[ref] = sum over the group's (query, neighbour) pairs of
  * 1 / (log1p(d) + 1) (or fixed_weight), scaled by multiplier / (n_g * k), then log1p.
  * Accumulated in 2^-40 fixed point with integer atomics: order-independent and
- * deterministic.  groups NULL: one group.  out FP64 [n_groups x n_ref].
+ * deterministic.  Target tables too large for one 64-bit fixed-point sum (more than
+ * ~1.6e7 / k rows at weight <= 1) are taken in batches whose sums are added in FP64 in
+ * batch order.  Distances are squared L2 (>= 0).  groups NULL: one group.
+ * out FP64 [n_groups x n_ref].
  * Replaces: mapping_datastore.py:211-285 (get_mapping_score). */
 int sh_mapping_score(const int64_t *idx, const float *dist, int64_t nq, int32_t k,
                      const int32_t *groups, int32_t n_groups, int64_t n_ref,

@@ -48,10 +48,10 @@ def coral(source: np.ndarray, target: np.ndarray):
     return s @ A, A
 
 
-def mapped_embedding(query_aligned: np.ndarray, reference: np.ndarray, mu, sigma, loadings,
-                     disable_scaling: bool = False) -> np.ndarray:
-    """The query's embedding under run_mapping(run_coral=True)."""
+def mapped_embedding(query_aligned: np.ndarray, reference: np.ndarray, loadings) -> np.ndarray:
+    """The query's embedding under run_mapping(run_coral=True): scarf turns feature
+    scaling off when CORAL runs (``mapping_datastore.py:141-142``), so make_graph hands
+    back an AnnStream with disable_scaling=True (``graph_datastore.py:886``) whose reducer
+    is ``data_coral @ loadings`` (``ann.py:126-127``)."""
     data, _ = coral(query_aligned, reference)
-    if disable_scaling:
-        return data @ loadings
-    return ((data - mu) / sigma) @ loadings
+    return data @ loadings

@@ -66,3 +66,54 @@ def transform_z_clipped(x, mu, sigma, clip: Optional[float]):
     if clip is not None:
         z = np.clip(z, -clip, clip)
     return z
+
+
+def csr_libsize(vals, cols, rowptr, n_cols, feat_idx=None):
+    """``libsize`` over the selected features of CSR counts (FP64 integer sums, exact)."""
+    n = rowptr.shape[0] - 1
+    rows = np.repeat(np.arange(n, dtype=np.int64), np.diff(rowptr))
+    keep = _feature_mask(cols, n_cols, feat_idx)
+    ls = np.bincount(rows[keep], weights=np.asarray(vals, dtype=np.float64)[keep], minlength=n)
+    ls[ls == 0] = 1
+    return ls
+
+
+def _feature_mask(cols, n_cols, feat_idx):
+    if feat_idx is None:
+        return np.ones(cols.shape[0], dtype=bool)
+    sel = np.zeros(n_cols, dtype=bool)
+    sel[np.asarray(feat_idx)] = True
+    return sel[cols]
+
+
+def csr_mu_sigma(vals, cols, rowptr, n_cols, sf=1000.0, log_transform=True):
+    """``mu_sigma(normed(...))`` for CSR counts over all features, without a dense copy:
+    zeros contribute x = 0, so mu_j = sum_nz x / n and the two-pass variance is
+    (sum_nz (x - mu_j)^2 + (n - nnz_j) mu_j^2) / n (ddof 0), then clean_array."""
+    n = rowptr.shape[0] - 1
+    ls = csr_libsize(vals, cols, rowptr, n_cols)
+    rows = np.repeat(np.arange(n, dtype=np.int64), np.diff(rowptr))
+    v = (float(sf) * np.asarray(vals, dtype=np.float64)) / ls[rows]
+    x = np.log1p(v) if log_transform else v
+    cols = np.asarray(cols, dtype=np.int64)
+    s1 = np.bincount(cols, weights=x, minlength=n_cols)
+    mu = s1 / n
+    nnz_j = np.bincount(cols, minlength=n_cols)
+    dev = x - mu[cols]
+    m2 = np.bincount(cols, weights=dev * dev, minlength=n_cols) + (n - nnz_j) * mu * mu
+    return clean_array(mu), clean_array(np.sqrt(m2 / n), 1)
+
+
+def csr_z_rows(vals, cols, rowptr, n_cols, rows, mu, sigma, sf=1000.0, clip=None,
+               log_transform=True, libsize=None):
+    """Dense FP64 z = clip((x - mu) / sigma) of the listed rows of CSR counts (all
+    features); ``libsize`` (per row of the whole matrix) defaults to the row sums."""
+    rows = np.asarray(rows, dtype=np.int64)
+    out = np.zeros((rows.shape[0], n_cols), dtype=np.float64)
+    for t, r in enumerate(rows):
+        a, b = int(rowptr[r]), int(rowptr[r + 1])
+        c = np.asarray(vals[a:b], dtype=np.float64)
+        ls = float(libsize[r]) if libsize is not None else (c.sum() or 1.0)
+        v = float(sf) * c / ls
+        out[t, np.asarray(cols[a:b], dtype=np.int64)] = np.log1p(v) if log_transform else v
+    return transform_z_clipped(out, mu, sigma, clip)

@@ -21,6 +21,7 @@ from typing import Optional
 
 import numpy as np
 
+from . import _lib
 from .device import DeviceCSR, col_map_tensor, ptr, require_gpu, stream_handle
 from .kmeans import MiniBatchKMeans
 from .knn import GpuIndex, fix_knn_query_device
@@ -33,6 +34,10 @@ except ImportError:  # pragma: no cover
     torch = None
 
 logger = logging.getLogger("scarf_amd")
+
+
+def _dense_code(X) -> int:
+    return _lib.SH_F64 if X.dtype == torch.float64 else _lib.SH_F32
 
 __all__ = ["AnnStream", "NormedCSR", "instantiate_knn_index", "fix_knn_query"]
 
@@ -240,25 +245,90 @@ class AnnStream:
         for b in self.data.blocks:
             yield b
 
-    def transform_z(self, a) -> np.ndarray:
-        """z of a block (dense FP64), ``ann.py:191-192``."""
+    def transform_z(self, a):
+        """z = (a - mu) / sigma of a block, ``ann.py:191-192``: a NormedCSR view (its rows
+        densified from the counts), a dense numpy block of normalised values (FP64 numpy
+        out) or a dense device tensor (FP64 tensor out)."""
         if isinstance(a, NormedCSR):
             c = torch.from_numpy(np.ascontiguousarray(self.mu)).cuda()
             s = torch.from_numpy(np.ascontiguousarray(1.0 / self.sigma)).cuda()
             return a.dense(c, s)
-        raise TypeError("transform_z expects a NormedCSR block")
+        X, host = self._dense_block(a)
+        ops = self._dense_ops()
+        z = torch.empty((max(X.shape[0], 1), X.shape[1]), dtype=torch.float64, device=X.device)
+        _lib.call("sh_dense_z", ptr(X), _dense_code(X), X.shape[0], X.shape[1], X.stride(0),
+                  ptr(ops["mu"]), ptr(ops["sigma"]), ops["clip"], ptr(z), z.stride(0),
+                  stream_handle())
+        z = z[:X.shape[0]]
+        return z.cpu().numpy() if host else z
 
-    def reducer_tensor(self, a: NormedCSR) -> "torch.Tensor":
+    def reducer_tensor(self, a) -> "torch.Tensor":
+        """Embedding of a block on the device: FP32 for a NormedCSR view (sparse
+        projection), FP64 for a dense block (``sh_dense_project``)."""
+        if not isinstance(a, NormedCSR):
+            X, _ = self._dense_block(a)
+            ops = self._dense_ops()
+            D = int(ops["W"].shape[1])
+            out = torch.empty((max(X.shape[0], 1), D), dtype=torch.float64, device=X.device)
+            _lib.call("sh_dense_project", ptr(X), _dense_code(X), X.shape[0], X.shape[1],
+                      X.stride(0), ptr(ops["mu"]) if ops["scaled"] else None,
+                      ptr(ops["sigma"]) if ops["scaled"] else None,
+                      ops["clip"] if ops["scaled"] else 0.0, ptr(ops["W"]), D, ptr(out),
+                      out.stride(0), stream_handle())
+            return out[:X.shape[0]]
         st = a.stats()
         E = self._proj.project(a.counts, a.norm, libsize=st.libsize, col_map=a._cm,
                                row_start=a.row_start, n_rows=a.row_end - a.row_start)
         return E
 
-    def reducer(self, a) -> np.ndarray:
-        """E = z @ loadings of a block as FP64 host array (``ann.py:129-138``)."""
+    def reducer(self, a):
+        """E = transform_z(a) @ loadings (or a @ loadings without scaling) of a block,
+        ``ann.py:119-138``: FP64 numpy for a NormedCSR view or a numpy block, an FP64
+        device tensor for a dense device tensor."""
         if isinstance(a, NormedCSR):
             return self.reducer_tensor(a).cpu().numpy().astype(np.float64)
-        raise TypeError("reducer expects a NormedCSR block")
+        E = self.reducer_tensor(a)
+        return E if isinstance(a, torch.Tensor) else E.cpu().numpy()
+
+    def _dense_block(self, a):
+        """(2-D FP32/FP64 device tensor with unit column stride, came-from-host flag)."""
+        host = not isinstance(a, torch.Tensor)
+        if host:
+            arr = np.asarray(a)
+            if arr.ndim == 1:
+                arr = arr[None, :]
+            if arr.dtype != np.float32:
+                arr = arr.astype(np.float64, copy=False)
+            X = torch.from_numpy(np.ascontiguousarray(arr)).cuda()
+        else:
+            X = a if a.dim() == 2 else a.reshape(1, -1)
+            if X.dtype not in (torch.float32, torch.float64):
+                X = X.to(torch.float64)
+            X = X.cuda()
+            if X.stride(1) != 1:
+                X = X.contiguous()
+        if X.shape[1] != self.nFeats:
+            raise ValueError(f"block has {X.shape[1]} features, the reference has {self.nFeats}")
+        return X, host
+
+    def _dense_ops(self):
+        """Device FP64 loadings, mu and sigma for dense blocks, rebuilt when the host
+        attributes change (scarf's mapping replaces mu / sigma, mapping_datastore.py:176-190)."""
+        mu = np.asarray(self.mu, dtype=np.float64)
+        sg = np.asarray(self.sigma, dtype=np.float64)
+        L = np.asarray(self.loadings, dtype=np.float64)
+        c = getattr(self, "_dense_cache", None)
+        if c is not None and np.array_equal(c["h_mu"], mu) and np.array_equal(c["h_sigma"], sg) \
+                and c["h_W"].shape == L.shape and np.array_equal(c["h_W"], L):
+            return c
+        dev = self.data.counts.rowptr.device
+        t = lambda v: torch.from_numpy(np.ascontiguousarray(v, dtype=np.float64)).to(dev)
+        scaled = self.method == "pca" and not self._disable_scaling
+        clip = float(self.data.norm.clip) if self.data.norm.clip else 0.0
+        c = {"h_mu": mu.copy(), "h_sigma": sg.copy(), "h_W": L.copy(), "mu": t(mu),
+             "sigma": t(sg), "W": t(L), "scaled": scaled, "clip": clip}
+        self._dense_cache = c
+        return c
 
     def transform_ann(self, a, k: int = None, self_indices=None):
         """``ann.py:194-205``: k neighbours, or k+1 then the self repair."""
@@ -277,10 +347,16 @@ class AnnStream:
     # -- ann.py:207-256 ------------------------------------------------------------------
     def _fit_pca(self, disable_scaling, use_for_pca, pca_mode="ipca", gram_precision="fp64"):
         st = self.data.stats()
+        center = scale = None
+        if not disable_scaling:  # z with the mu / sigma this object holds (ann.py:228-229)
+            dev = self.data.counts.rowptr.device
+            center = torch.from_numpy(np.ascontiguousarray(self.mu, dtype=np.float64)).to(dev)
+            scale = torch.from_numpy(
+                np.ascontiguousarray(1.0 / np.asarray(self.sigma, dtype=np.float64))).to(dev)
         fit = fit_pca(self.data.counts, st, self.data.norm, self.dims, self.batchSize,
                       use_for_pca=None if use_for_pca is None else np.asarray(use_for_pca, bool),
                       mode=pca_mode, col_map=self.data._cm, disable_scaling=disable_scaling,
-                      precision=gram_precision)
+                      precision=gram_precision, center=center, scale=scale)
         self._pca = fit.state
         self.loadings = fit.loadings
 

@@ -4,8 +4,9 @@
 scarf standardises the aligned query data by its own feature means and standard
 deviations (``clean_array``, ddof 0), the reference data by its own, takes both feature
 covariances (ddof 1, plus the identity), and rewrites the query as
-``data_coral = s @ C_s^{-1/2} C_t^{1/2}``; the mapping then reduces ``data_coral`` with the
-reference's ``transform_z`` and loadings (``mapping_datastore.py:176-208``).
+``data_coral = s @ C_s^{-1/2} C_t^{1/2}``.  run_mapping turns feature scaling off when CORAL
+runs (``mapping_datastore.py:141-142``), so the reference AnnStream reduces ``data_coral``
+with its loadings alone, ``data_coral @ L`` (``graph_datastore.py:886``, ``ann.py:126-127``).
 
 Here nothing of size cells x features is formed:
 
@@ -14,13 +15,13 @@ Here nothing of size cells x features is formed:
   (``sh_cov_from_gram``); the reference's is computed once per ``AnnStream``;
 * C_s^{-1/2} and C_t^{1/2} come from the coupled Newton-Schulz iteration on FP64 GEMMs
   (``sh_spd_sqrt``) instead of scipy's ``fractional_matrix_power``;
-* the whole map x -> ((s @ A) - mu) / sigma @ L is linear in x, so it folds into the sparse
-  projection kernel: weights B = A diag(1/sigma) L (an FP64 GEMM) over s = (x - sm) / sd, and
-  the constant -(mu / sigma) @ L.
+* the whole map x -> (s @ A) @ L is linear in x, so it folds into the sparse projection
+  kernel: weights B = A L (an FP64 GEMM) over s = (x - sm) / sd.
 """
 from __future__ import annotations
 
 import dataclasses
+import logging
 
 import numpy as np
 
@@ -33,6 +34,8 @@ try:
     import torch
 except ImportError:  # pragma: no cover
     torch = None
+
+logger = logging.getLogger("scarf_amd")
 
 __all__ = ["spd_sqrt", "covariance_plus_identity", "coral_projector", "CoralInfo"]
 
@@ -125,13 +128,14 @@ def coral_projector(ann, tcsr, col_map, missing, norm, n_counts=None):
     _, S_mh, its, rs = spd_sqrt(S, sqrt=False, inv_sqrt=True)
     T_h, _, itt, rt = spd_sqrt(T, sqrt=True, inv_sqrt=False)
     A = _dgemm(S_mh, T_h)
+    # scarf runs CORAL with feature scaling off (mapping_datastore.py:141-142): the
+    # reference AnnStream then reduces data_coral @ loadings with no z-scoring
+    # (graph_datastore.py:886, ann.py:126-127), whatever the graph was built with
     L = np.asarray(ann.loadings, dtype=np.float64)
-    if ann._disable_scaling:
-        Lz = L
-        offset = np.zeros(L.shape[1])
-    else:
-        Lz = L / np.asarray(ann.sigma, dtype=np.float64)[:, None]
-        offset = -(np.asarray(ann.mu, dtype=np.float64) / np.asarray(ann.sigma)) @ L
-    B = _dgemm(A, torch.from_numpy(np.ascontiguousarray(Lz)).to(dev)).cpu().numpy()
-    proj = Projector.build(B, sm, sd, clip=None, fill_missing=missing, offset=offset)
+    if not ann._disable_scaling:
+        logger.warning("run_coral: scarf maps CORAL-corrected data with feature scaling "
+                       "off; this reference AnnStream was fitted with feat_scaling=True, "
+                       "so its loadings are applied to the unscaled corrected data")
+    B = _dgemm(A, torch.from_numpy(np.ascontiguousarray(L)).to(dev)).cpu().numpy()
+    proj = Projector.build(B, sm, sd, clip=None, fill_missing=missing)
     return proj, CoralInfo(its, rs, itt, rt)

@@ -723,10 +723,12 @@ __global__ void __launch_bounds__(256) k_rerank(
   // element e is at lane e/PE, slot e%PE
   int32_t flag = 0;
   // certificate: every reference outside the pool has exact distance
-  // >= lastkey + ||q||^2 - err; the k-th chosen one must be strictly below that
+  // >= lastkey + ||q||^2 - err; the k-th chosen one must be strictly below that.
+  // References were left out whenever the index holds more rows than the merged pool
+  // (each slice stores at most `pool` keys, so `total` cannot tell with one slice).
   if (err_coef >= 0.0) {
     bool certified = true;
-    if (total > pool && kk > 0) {
+    if (nr > (int64_t)npool && kk > 0) {
       double qn = 0.0;
       for (int t = 0; t < D; ++t) qn += (double)ql[t] * (double)ql[t];
       const double rq = sqrt(qn);
@@ -892,7 +894,8 @@ __global__ void __launch_bounds__(256) k_exact(
       if (dist64_out) dist64_out[q * k + col] = de;
     }
   }
-  if (flags && lane == 0) flags[q] = flag;
+  // a row recomputed here after the re-rank's certificate failed keeps that flag
+  if (flags && lane == 0) flags[q] = (flags[q] & SH_FLAG_UNCERTIFIED) | flag;
 }
 
 

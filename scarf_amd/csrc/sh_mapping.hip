@@ -45,16 +45,30 @@ __global__ void k_ms_scatter(const int64_t *__restrict__ idx, const float *__res
   atomicAdd((unsigned long long *)&acc[(int64_t)g * n_ref + r], (unsigned long long)f);
 }
 
-__global__ void k_ms_final(const uint64_t *__restrict__ acc, const int32_t *__restrict__ cnt,
-                           int32_t n_groups, int64_t n_ref, int32_t k, double multiplier,
-                           int log_t, double *__restrict__ out) {
+// acc (fixed point) -> ms.  With `sums` non-NULL the target rows came in several
+// batches: `sums` holds the FP64 sum of the earlier batches' fixed-point totals and
+// acc the last batch's; they are added in batch order.
+__global__ void k_ms_final(const uint64_t *__restrict__ acc, const double *__restrict__ sums,
+                           const int32_t *__restrict__ cnt, int32_t n_groups, int64_t n_ref,
+                           int32_t k, double multiplier, int log_t, double *__restrict__ out) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (int64_t)n_groups * n_ref) return;
   const int g = (int)(t / n_ref);
-  const double s = (double)acc[t] / kFix;
+  double s = (double)acc[t] / kFix;
+  if (sums) s = sums[t] + s;
   const double den = (double)cnt[g] * (double)k;
   double ms = multiplier * s / den;
   out[t] = log_t ? log1p(ms) : ms;
+}
+
+// Between batches: sums (+)= acc / 2^40, acc = 0.
+__global__ void k_ms_fold(uint64_t *__restrict__ acc, double *__restrict__ sums, int64_t m,
+                          int first) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= m) return;
+  const double s = (double)acc[t] / kFix;
+  sums[t] = first ? s : sums[t] + s;
+  acc[t] = 0;
 }
 
 
@@ -150,9 +164,12 @@ extern "C" int sh_mapping_score(const int64_t *idx, const float *dist, int64_t n
   SH_ARG(nq >= 0 && k >= 1 && n_ref >= 1 && n_groups >= 1, "sh_mapping_score: bad sizes");
   SH_ARG(groups || n_groups == 1, "sh_mapping_score: groups NULL requires n_groups == 1");
   SH_ARG(idx && dist && out, "sh_mapping_score: NULL pointer");
-  SH_ARG(weighted || (fixed_weight > 0 && fixed_weight * kFix * nq * k < 1.8e19),
+  // one (query, neighbour) pair adds at most w_max (1 for weighted with d >= 0) to a
+  // reference's 64-bit fixed-point sum; a batch of nb rows adds at most nb * k * w_max
+  const double w_max = weighted ? 1.0 : fixed_weight;
+  SH_ARG(weighted || (fixed_weight > 0 && fixed_weight * kFix * k < 1.8e19),
          "sh_mapping_score: fixed_weight out of range");
-  SH_ARG((double)nq * k * kFix < 1.8e19, "sh_mapping_score: too many pairs for 2^-40 fixed point");
+  const int64_t nb = (int64_t)(1.8e19 / (kFix * (double)k * (w_max > 1.0 ? w_max : 1.0)));
   const size_t need = sh_mapping_score_workspace(n_ref, n_groups);
   if (!workspace || workspace_bytes < need) {
     set_error("sh_mapping_score: workspace %zu < %zu bytes", workspace_bytes, need);
@@ -166,13 +183,23 @@ extern "C" int sh_mapping_score(const int64_t *idx, const float *dist, int64_t n
   hipLaunchKernelGGL(k_ms_clear, dim3(cdiv(m > n_groups ? m : n_groups, 256)), dim3(256), 0, st,
                      acc, m, cnt, n_groups);
   SH_LAUNCH_CHECK("k_ms_clear");
-  if (nq > 0) {
-    hipLaunchKernelGGL(k_ms_scatter, dim3(cdiv(nq * k, 256)), dim3(256), 0, st, idx, dist, nq, k,
-                       groups, n_groups, n_ref, weighted, fixed_weight, acc, cnt);
+  // batches of nb target rows; `out` holds the running FP64 sums between batches
+  const double *sums = nullptr;
+  for (int64_t a = 0; a < nq; a += nb) {
+    const int64_t b = nq - a < nb ? nq - a : nb;
+    if (a > 0) {
+      hipLaunchKernelGGL(k_ms_fold, dim3(cdiv(m, 256)), dim3(256), 0, st, acc, out, m,
+                         (int)(a == nb));
+      SH_LAUNCH_CHECK("k_ms_fold");
+      sums = out;
+    }
+    hipLaunchKernelGGL(k_ms_scatter, dim3(cdiv(b * k, 256)), dim3(256), 0, st, idx + a * k,
+                       dist + a * k, b, k, groups ? groups + a : nullptr, n_groups, n_ref,
+                       weighted, fixed_weight, acc, cnt);
     SH_LAUNCH_CHECK("k_ms_scatter");
   }
-  hipLaunchKernelGGL(k_ms_final, dim3(cdiv(m, 256)), dim3(256), 0, st, acc, cnt, n_groups, n_ref,
-                     k, multiplier, log_transform, out);
+  hipLaunchKernelGGL(k_ms_final, dim3(cdiv(m, 256)), dim3(256), 0, st, acc, sums, cnt, n_groups,
+                     n_ref, k, multiplier, log_transform, out);
   SH_LAUNCH_CHECK("k_ms_final");
   return SH_OK;
 }

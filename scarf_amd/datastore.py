@@ -183,8 +183,9 @@ def project_target(ann: AnnStream, target_counts, target_ids: Sequence,
     The target uses the reference's size factor and subset parameters
     (``mapping_datastore.py:106-111``, ``mapping_utils.py:188-194``); features are aligned
     to the reference order with missing ones set to 1.0 (``mapping_utils.py:197-213``).
-    ``run_coral`` applies CORAL first (``mapping_datastore.py:160-175``), in which case the
-    reference's own mu/sigma are used whatever ``ref_mu`` / ``ref_sigma`` say, as in scarf.
+    ``run_coral`` applies CORAL first (``mapping_datastore.py:160-175``); scarf then maps with
+    feature scaling off (``mapping_datastore.py:141-142``), so the corrected data is reduced
+    as ``data_coral @ loadings`` with no mu/sigma, whatever ``ref_mu`` / ``ref_sigma`` say.
     """
     require_gpu()
     tcsr = _device_csr(target_counts)

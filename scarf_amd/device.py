@@ -116,6 +116,20 @@ class DeviceCSR:
         if self.cols.numel() and (bool((self.cols < 0).any()) or
                                   bool((self.cols >= self.n_cols).any())):
             raise ValueError("CSR column index out of range")
+        nnz = self.cols.numel()
+        if nnz > 1:
+            # fast path: columns strictly increasing inside every row
+            step = self.cols[1:] > self.cols[:-1]
+            starts = rp[1:-1]
+            starts = starts[(starts > 0) & (starts < nnz)]
+            step[starts - 1] = True
+            if not bool(step.all()):
+                rows = torch.repeat_interleave(
+                    torch.arange(rp.numel() - 1, device=rp.device), rp[1:] - rp[:-1])
+                keys = rows * self.n_cols + self.cols.to(torch.int64)
+                if torch.unique(keys).numel() != nnz:
+                    raise ValueError("CSR has a column repeated within a row (sum duplicate "
+                                     "entries before upload)")
 
     def to_host(self):
         from .synth import CSR
@@ -132,6 +146,19 @@ def validate_csr_host(vals, cols, rowptr, n_cols):
         raise ValueError("CSR row pointers are not monotone")
     if cols.shape[0] and (cols.min() < 0 or cols.max() >= n_cols):
         raise ValueError("CSR column index out of range")
+    nnz = cols.shape[0]
+    if nnz > 1:
+        # fast path: columns strictly increasing inside every row
+        step = cols[1:] > cols[:-1]
+        starts = np.asarray(rowptr[1:-1])
+        starts = starts[(starts > 0) & (starts < nnz)]
+        step[starts - 1] = True
+        if not step.all():
+            rows = np.repeat(np.arange(rowptr.shape[0] - 1, dtype=np.int64), np.diff(rowptr))
+            keys = rows * int(n_cols) + cols.astype(np.int64)
+            if np.unique(keys).shape[0] != nnz:
+                raise ValueError("CSR has a column repeated within a row (sum duplicate "
+                                 "entries before upload)")
 
 
 def col_map_tensor(n_cols_in: int, feat_idx=None, device="cuda"):

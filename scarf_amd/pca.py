@@ -264,10 +264,13 @@ def gram_of_rows(csr: DeviceCSR, stats: FeatureStats, params: NormParams, rows,
 def fit_pca(csr: DeviceCSR, stats: FeatureStats, params: NormParams, dims: int,
             chunk_rows: int, use_for_pca: Optional[np.ndarray] = None, mode: str = "ipca",
             col_map=None, disable_scaling: bool = False, precision: str = "fp64",
-            eig: str = "auto", stream=None) -> PcaFit:
+            eig: str = "auto", stream=None, center=None, scale=None) -> PcaFit:
     """Fits dims+1 components and keeps the first ``dims`` as loadings (ann.py:212,256).
 
     ``eig`` selects the eigensolver of each partial fit (see ``IPCAState.merge``).
+    z = (x - center) * scale; ``center`` / ``scale`` (device FP64 [d]) default to the
+    stats' mu and 1 / sigma (AnnStream passes the mu / sigma it was given, as scarf's
+    ``transform_z`` uses ``self.mu`` / ``self.sigma``, ``ann.py:191-192, 228-229``).
     """
     require_gpu()
     n = csr.n_rows
@@ -287,7 +290,8 @@ def fit_pca(csr: DeviceCSR, stats: FeatureStats, params: NormParams, dims: int,
     def fit_call(rows_np):
         rows = torch.from_numpy(np.ascontiguousarray(rows_np)).to(dev)
         G, cs, m = gram_of_rows(csr, stats, params, rows, col_map, disable_scaling,
-                                precision=precision, stream=stream, to_host=eig == "host")
+                                precision=precision, stream=stream, to_host=eig == "host",
+                                center=center, scale=scale)
         state.merge(m, G, cs, eig=eig, stream=stream)
 
     if mode == "exact":

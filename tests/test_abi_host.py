@@ -110,6 +110,25 @@ def test_csr_validation_host():
     validate_csr_host(v, np.array([0, 1, 2], np.int32), np.array([0, 2, 3]), 4)
 
 
+def test_csr_validation_rejects_repeated_columns():
+    """The kernels take each (row, column) at most once; a repeated column would race on
+    one accumulator and be normalised twice instead of summed first."""
+    from scarf_amd.device import validate_csr_host
+    v = np.array([1, 2, 3, 4], np.int32)
+    rp = np.array([0, 2, 4])
+    with pytest.raises(ValueError, match="repeated"):
+        validate_csr_host(v, np.array([1, 1, 0, 2], np.int32), rp, 4)  # sorted duplicate
+    with pytest.raises(ValueError, match="repeated"):
+        validate_csr_host(v, np.array([0, 1, 3, 3], np.int32), rp, 4)
+    with pytest.raises(ValueError, match="repeated"):
+        validate_csr_host(np.ones(3, np.int32), np.array([2, 0, 2], np.int32),
+                          np.array([0, 3]), 4)  # unsorted duplicate
+    # unsorted rows without repeats, and a column equal across a row boundary, pass
+    validate_csr_host(v, np.array([3, 1, 0, 2], np.int32), rp, 4)
+    validate_csr_host(v, np.array([0, 3, 3, 1], np.int32), rp, 4)
+    validate_csr_host(v, np.array([0, 3, 1, 2], np.int32), np.array([0, 2, 2, 4]), 4)
+
+
 def test_slice_plan():
     from scarf_amd.knn import _fix_slices, default_slices
     for nr in (64, 100, 5000, 500_000, 1_000_000):

@@ -36,8 +36,9 @@ def test_spd_sqrt_vs_scipy(cuda, d, rank):
 @pytest.mark.parametrize("scaling", [True, False])
 def test_coral_mapping_embedding(cuda, scaling):
     """project_target(run_coral=True) against the dense restatement: query and reference
-    standardised by their own stats, C_s^{-1/2} C_t^{1/2}, then the reference's z (or no z
-    with feat_scaling=False) and loadings.  FP32 projection: 1e-4 of the embedding's scale."""
+    standardised by their own stats, C_s^{-1/2} C_t^{1/2}, then the loadings with no z-scoring
+    (scarf maps CORAL-corrected data with feature scaling off, whatever the reference was
+    built with).  FP32 projection: 1e-4 of the embedding's scale."""
     from scarf_amd.datastore import make_graph, project_target, run_mapping
     from scarf_amd.synth import CSR, generate_csr, make_gene_model
     model = make_gene_model(240, 4, 0.12, 41)
@@ -61,8 +62,7 @@ def test_coral_mapping_embedding(cuda, scaling):
     t_idx = omap.order_features(src_ids, tids)
     xa = omap.align_normed(tcounts, t_idx, sf=1000.0)
     xr = onorm.normed(csr.to_dense(), sf=1000.0, feat_idx=feat)
-    want = oc.mapped_embedding(xa, xr, ann.mu, ann.sigma, ann.loadings,
-                               disable_scaling=not scaling)
+    want = oc.mapped_embedding(xa, xr, ann.loadings)
     err = np.abs(E - want).max() / np.abs(want).max()
     say("CORAL embedding max error / scale", err)
     assert err < 1e-4

@@ -124,13 +124,16 @@ def test_run_mapping_and_scores(cuda, ref_data):
     xa = omap.align_normed(tcounts, t_idx, sf=1000.0)
     Et = ((xa - ann.mu) / ann.sigma) @ ann.loadings
     Ef = ann.annIdx.embedding().cpu().numpy()
-    # GPU's target embedding vs oracle (FP32 accumulation)
-    from scarf_amd.pca import Projector
+    # the GPU's target embedding (FP32 projection) against the oracle's FP64 one
+    from scarf_amd.datastore import project_target
+    Eg = project_target(ann, CSR.from_dense(tcounts), tids, src_ids).cpu().numpy()
+    assert np.abs(Eg - Et).max() / np.abs(Et).max() < 1e-5
+    # neighbours: bit for bit against the FP64 brute force on that same embedding
     i_got = store["RNA/projections/t1/indices"].astype(np.int64)
-    ri, rd = oknn.knn_f64(Et.astype(np.float32), Ef, 5)
-    agree = np.mean([len(set(i_got[i]) & set(ri[i])) / 5 for i in range(len(ri))])
-    say("mapping neighbours agree with oracle-projected targets:", agree)
-    assert agree > 0.99
+    ri, rd = oknn.knn_f64(Eg, Ef, 5)
+    np.testing.assert_array_equal(i_got, ri)
+    np.testing.assert_array_equal(store["RNA/projections/t1/distances"],
+                                  rd.astype(np.float32).astype(np.float64))
     # scores: oracle formula on the stored table
     (_, ms), = list(get_mapping_score(store, "t1"))
     (_, msr), = omap.mapping_scores(i_got, store["RNA/projections/t1/distances"], ann.nCells)
@@ -236,3 +239,103 @@ def test_transform_ann_noncontiguous_self(cuda, ref_data):
     fi, fd, fnm = oknn.fix_knn_query(ki, kd, rows)
     np.testing.assert_array_equal(i.astype(np.int64), fi)
     assert nm == fnm
+
+
+@pytest.mark.parametrize("clip", [None, 3.0])
+def test_dense_block_reducer_mapping_loop(cuda, ref_data, clip):
+    """scarf's own mapping loop on dense numpy blocks of normalised values
+    (mapping_datastore.py:198-205): ki, kd = ann.transform_ann(ann.reducer(a), k=save_k).
+    transform_z equals numpy's (a - mu) / sigma bit for bit; the reducer is the FP64
+    product within rounding; the neighbours equal the FP64 brute force on that embedding."""
+    import torch
+    from scarf_amd.datastore import make_graph
+    model, csr = ref_data
+    feat = np.arange(0, 300, 2)
+    ann = make_graph(csr, feat, dims=15, k=11, batch_size=600, screen="fp32", clip=clip,
+                     return_ann_object=True)
+    Ef = ann.annIdx.embedding().cpu().numpy()
+    tgt = generate_csr(700, model, 79, batch_shift_sigma=0.2).to_dense()[:, feat]
+    x = onorm.normed(tgt, sf=1000.0)
+    say("dense-block mapping loop, clip", clip)
+    for s in range(0, 700, 250):
+        a = x[s:s + 250]
+        z_ref = (a - ann.mu) / ann.sigma
+        if clip:
+            z_ref = np.clip(z_ref, -clip, clip)
+        z = ann.transform_z(a)
+        assert isinstance(z, np.ndarray) and z.dtype == np.float64
+        np.testing.assert_array_equal(z, z_ref)
+        E = ann.reducer(a)
+        assert isinstance(E, np.ndarray) and E.dtype == np.float64 and E.shape == (len(a), 15)
+        Er = z_ref @ ann.loadings
+        scale = np.abs(z_ref) @ np.abs(ann.loadings)
+        assert (np.abs(E - Er) / scale).max() < 1e-13
+        ki, kd = ann.transform_ann(E, k=5)
+        ri, rd = oknn.knn_f64(E.astype(np.float32), Ef, 5)
+        np.testing.assert_array_equal(ki.astype(np.int64), ri)
+        np.testing.assert_array_equal(kd, rd.astype(np.float32))
+    # FP32 blocks and device tensors take the same kernel
+    a = x[:100]
+    np.testing.assert_array_equal(ann.reducer(a.astype(np.float32)),
+                                  ann.reducer(a.astype(np.float32).astype(np.float64)))
+    Et = ann.reducer(torch.from_numpy(a).cuda())
+    assert isinstance(Et, torch.Tensor) and Et.dtype == torch.float64
+    np.testing.assert_array_equal(Et.cpu().numpy(), ann.reducer(a))
+    np.testing.assert_array_equal(ann.transform_z(torch.from_numpy(a).cuda()).cpu().numpy(),
+                                  ann.transform_z(a))
+    with pytest.raises(ValueError):
+        ann.reducer(x[:10, :-1])
+    # mapping replaces mu / sigma (ref_mu / ref_sigma False, mapping_datastore.py:176-190)
+    ann.mu = x.mean(0)
+    ann.sigma = onorm.clean_array(x.std(0), 1)
+    z_ref = (a - ann.mu) / ann.sigma
+    if clip:
+        z_ref = np.clip(z_ref, -clip, clip)
+    np.testing.assert_array_equal(ann.transform_z(a), z_ref)
+
+
+def test_dense_block_reducer_without_scaling(cuda, ref_data):
+    """feat_scaling=False (disable_scaling, ann.py:119-127): reducer(a) = a @ loadings."""
+    from scarf_amd.datastore import make_graph
+    model, csr = ref_data
+    ann = make_graph(csr, None, dims=10, k=7, batch_size=800, screen="fp32",
+                     feat_scaling=False, return_ann_object=True)
+    a = onorm.normed(generate_csr(120, model, 80).to_dense(), sf=1000.0)
+    E = ann.reducer(a)
+    Er = a @ ann.loadings
+    assert (np.abs(E - Er) / (np.abs(a) @ np.abs(ann.loadings))).max() < 1e-13
+
+
+def test_batch_size_and_k_clamps(cuda):
+    """ann.py:85-86 (k >= nCells -> nCells - 1) and _handle_batch_size, ann.py:173-185
+    (dims >= batch_size -> batch_size - 1, nClusters > batch_size -> batch_size), with the
+    clamped graph checked against the oracle."""
+    from scarf_amd.datastore import make_graph
+    model = make_gene_model(60, 2, 0.3, 51)
+    csr = generate_csr(40, model, 51)
+    say("clamps: 40 cells, k=100, dims=30, batch_size=20")
+    store = {}
+    ann = make_graph(csr, None, dims=30, k=100, batch_size=20, screen="fp32", store=store,
+                     return_ann_object=True)
+    assert ann.k == 39
+    assert ann.dims == 19 and ann.loadings.shape == (60, 19)
+    assert ann.nClusters == 20
+    x = onorm.normed(csr.to_dense(), sf=1000.0)
+    mu, sigma = onorm.mu_sigma(x)
+    blocks = [x[s:s + 20] for s in range(0, 40, 20)]
+    L = _sk_loadings(x, blocks, 19, mu, sigma)
+    E = ann.annIdx.embedding().cpu().numpy().astype(np.float64)
+    Er = ((x - mu) / sigma) @ L
+    for j in range(19):
+        sg = np.sign(E[:, j] @ Er[:, j])
+        assert np.linalg.norm(sg * E[:, j] - Er[:, j]) / np.linalg.norm(Er[:, j]) < 1e-4
+    fi, fd, _, _ = oknn.self_query_knn(E.astype(np.float32), 39)
+    knn_loc = store["latest_knn"]
+    np.testing.assert_array_equal(store[knn_loc + "/indices"], fi.astype(np.uint64))
+    # dims larger than the cell count is clamped to it (ann.py:174-175); fitting
+    # dims + 1 components on 15 cells then fails, as IncrementalPCA's does
+    small = generate_csr(15, model, 52)
+    with pytest.raises(ValueError):
+        make_graph(small, None, dims=30, k=5, batch_size=1000, screen="fp32",
+                   return_ann_object=True)
+

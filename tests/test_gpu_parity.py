@@ -188,6 +188,29 @@ def test_knn_fp32_slices_and_self(cuda):
         assert got_mis == drop
 
 
+@pytest.mark.parametrize("n_slices", [1, 4])
+def test_knn_fp32_uncertified_rows_fall_back(cuda, n_slices):
+    """Near-tie data with a pool smaller than the tie group: the re-rank cannot certify
+    those rows (flag 1), with one reference slice as with several, and their FP64
+    brute-force recomputation equals the oracle."""
+    import torch
+    from scarf_amd.knn import pack_refs, search
+    rng = np.random.default_rng(31)
+    R = rng.standard_normal((3000, 24)).astype(np.float32)
+    R[100:400] = R[100]  # 300 identical rows
+    R[500:540] = R[500] + rng.standard_normal((40, 24)).astype(np.float32) * 1e-6
+    Q = np.concatenate([R[[100, 250, 500, 520]], rng.standard_normal((60, 24)).astype(np.float32)])
+    Rt, Qt = torch.from_numpy(R).cuda(), torch.from_numpy(Q).cuda()
+    say("knn fp32 uncertified rows, slices", n_slices)
+    refs = pack_refs(Rt, "fp32")
+    idx, dist, flags = search(Qt, Rt, refs, 11, pool=16, n_slices=n_slices, order=False)
+    f = flags.cpu().numpy()
+    assert (f[:2] & 1).all(), f[:4]
+    ri, rd = oknn.knn_f64(Q, R, 11)
+    np.testing.assert_array_equal(idx.cpu().numpy(), ri)
+    np.testing.assert_array_equal(dist.cpu().numpy(), rd.astype(np.float32))
+
+
 def test_knn_bf16_recall(cuda):
     import torch
     from scarf_amd.knn import pack_refs, search
@@ -409,3 +432,29 @@ def test_target_classes_gpu(cuda, k, threshold):
                          torch.from_numpy(groups.astype(np.int32)), threshold).cpu().numpy()
     ref = ref_tc(idx, d, groups, threshold, na_val=-1)
     np.testing.assert_array_equal(got, np.array(ref))
+
+
+def test_mapping_scores_beyond_one_fixed_point_batch(cuda):
+    """More (query, neighbour) pairs than one 2^-40 fixed-point sum can hold at k = 11
+    (1.6M target rows, 17.6M pairs): the entry point takes them in batches and adds the
+    batch sums in FP64.  Against the oracle's per-group FP64 sums."""
+    import torch
+    from scarf_amd.mapping import mapping_scores
+    rng = np.random.default_rng(44)
+    nq, k, n_ref = 1_600_000, 11, 5000
+    idx = rng.integers(0, n_ref, (nq, k)).astype(np.int64)
+    dist = rng.uniform(0.0, 30.0, (nq, k)).astype(np.float32)
+    groups = rng.integers(0, 3, nq)
+    say("mapping scores", nq, "x", k, "pairs over", n_ref, "references, 3 groups")
+    got = mapping_scores(torch.from_numpy(idx).cuda(), torch.from_numpy(dist).cuda(), n_ref,
+                         groups=groups)
+    w = 1.0 / (np.log1p(dist.astype(np.float64)) + 1.0)
+    for g, ms in got:
+        sel = groups == g
+        s = np.bincount(idx[sel].ravel(), weights=w[sel].ravel(), minlength=n_ref)
+        want = np.log1p(1000.0 * s / (sel.sum() * k))
+        np.testing.assert_allclose(ms, want, rtol=1e-11)
+    got_fw = mapping_scores(torch.from_numpy(idx).cuda(), torch.from_numpy(dist).cuda(), n_ref,
+                            weighted=False, fixed_weight=0.1, log_transform=False)
+    s = np.bincount(idx.ravel(), minlength=n_ref) * 0.1
+    np.testing.assert_allclose(got_fw[0][1], 1000.0 * s / (nq * k), rtol=1e-12)

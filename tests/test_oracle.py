@@ -360,3 +360,62 @@ def test_coral_oracle_pins():
     data, A2 = oc.coral(x, x)  # same distribution: nothing to correct
     np.testing.assert_allclose(A2, np.eye(12), atol=1e-10)
     np.testing.assert_allclose(data, (x - x.mean(0)) / x.std(0), atol=1e-9)
+
+
+def test_csr_stats_match_dense():
+    """The sparse restatement used at full size equals the dense oracle (all-zero rows,
+    all-zero and constant features included)."""
+    from scarf_amd.synth import CSR, generate_csr, make_gene_model
+    csr = generate_csr(700, make_gene_model(90, 3, 0.15, 3), 3)
+    dense = csr.to_dense()
+    dense[5] = 0
+    dense[:, 7] = 0
+    dense[:, 11] = 4
+    c = CSR.from_dense(dense.astype(np.uint32))
+    for sf in (1000.0, 1e4):
+        x = norm.normed(dense, sf=sf)
+        mu, sigma = norm.mu_sigma(x)
+        m2, s2 = norm.csr_mu_sigma(c.vals, c.cols, c.rowptr, c.n_cols, sf=sf)
+        np.testing.assert_allclose(m2, mu, rtol=1e-13, atol=1e-16)
+        np.testing.assert_allclose(s2, sigma, rtol=1e-12, atol=1e-16)
+        assert s2[7] == 1.0 and m2[7] == 0.0
+        rows = np.array([0, 5, 699, 3])
+        z = norm.csr_z_rows(c.vals, c.cols, c.rowptr, c.n_cols, rows, mu, sigma, sf=sf, clip=3.0)
+        np.testing.assert_allclose(z, norm.transform_z_clipped(x[rows], mu, sigma, 3.0),
+                                   rtol=1e-14, atol=1e-15)
+    np.testing.assert_array_equal(norm.csr_libsize(c.vals, c.cols, c.rowptr, c.n_cols),
+                                  norm.libsize(dense))
+
+
+def test_c_stats_and_zgram_match_numpy():
+    """The C restatement used at full size: libsize, mu, sigma and Z^T Z with the column
+    sums equal the numpy oracle on the dense matrix (clip on and off, an all-zero row,
+    an all-zero feature, unsorted columns in one row)."""
+    from oracle import stats as ostats
+    from scarf_amd.synth import CSR, generate_csr, make_gene_model
+    csr = generate_csr(900, make_gene_model(70, 3, 0.2, 4), 4)
+    dense = csr.to_dense()
+    dense[3] = 0
+    dense[:, 9] = 0
+    c = CSR.from_dense(dense.astype(np.uint32))
+    a, b = int(c.rowptr[10]), int(c.rowptr[11])
+    c.vals[a:b], c.cols[a:b] = c.vals[a:b][::-1].copy(), c.cols[a:b][::-1].copy()
+    for sf, clip in ((1000.0, None), (1e4, 2.5)):
+        ls, mu, sg = ostats.csr_stats(c.vals, c.cols, c.rowptr, c.n_cols, sf=sf)
+        x = norm.normed(dense, sf=sf)
+        m0, s0 = norm.mu_sigma(x)
+        np.testing.assert_array_equal(ls, norm.libsize(dense))
+        np.testing.assert_allclose(mu, m0, rtol=1e-13, atol=1e-16)
+        np.testing.assert_allclose(sg, s0, rtol=1e-12)
+        G, cs = ostats.csr_zgram(c.vals, c.cols, c.rowptr, c.n_cols, ls, mu, sg, sf=sf, clip=clip)
+        z = norm.transform_z_clipped(x, mu, sg, clip)
+        scale = np.abs(z).T @ np.abs(z)
+        assert (np.abs(G - z.T @ z) / np.maximum(scale, 1e-300)).max() < 1e-13
+        assert not G[9].any() and not G[:, 9].any()  # the all-zero feature has z = 0
+        np.testing.assert_allclose(cs, z.sum(0), atol=1e-10)
+        w, Vt = ostats.exact_pca(G, cs, z.shape[0], 5)
+        zc = z - z.mean(0)
+        _, S, V2 = np.linalg.svd(zc, full_matrices=False)
+        np.testing.assert_allclose(w, S[:5] ** 2, rtol=1e-10)
+        V2 = ipca.svd_flip_rows(V2[:5])
+        np.testing.assert_allclose(Vt, V2, atol=1e-8)
