@@ -207,6 +207,132 @@ def recall_vs_exact(res, k, n, n_queries, seed=7):
             "kth_dist_rel_err_mean": float(err.mean()), "recall_queries": int(len(qs))}
 
 
+def streamed_step(csr, params, n):
+    """Untimed by the metric: one step whose counts start in pinned host memory and
+    stream to HBM chunk by chunk while the statistics run (the PCIe-inclusive rate)."""
+    import torch
+
+    from scarf_amd.graph import StageTimer, build_graph_streamed
+    from scarf_amd.normalize import HostCSR
+    host = HostCSR.pinned_from(csr.vals, csr.cols, csr.rowptr, csr.n_cols)
+    nbytes = host.vals.numel() * 8 + host.rowptr.numel() * 8
+    build_graph_streamed(host, params)  # warm
+    torch.cuda.synchronize()
+    timer = StageTimer(True)
+    t0 = time.perf_counter()
+    res = build_graph_streamed(host, params, timer=timer)
+    torch.cuda.synchronize()
+    ms = 1e3 * (time.perf_counter() - t0)
+    st = timer.result()
+    out = {"ms_per_step": ms, "cells_per_s": n / (ms * 1e-3),
+           "upload_stats_ms": st.get("upload_stats"), "host_bytes": nbytes,
+           "stage_ms": {k: round(v, 3) for k, v in st.items()},
+           "note": f"CSR in pinned host memory, copied in {params.batch_size}-row chunks on a "
+                   f"copy stream while each landed chunk's statistics run"}
+    log(f"streamed step: {ms:.1f} ms (upload + stats {out['upload_stats_ms']:.1f} ms)")
+    del res, host
+    return out
+
+
+def pca_vs_ipca(res, csr, params):
+    """Untimed: the same PCA fitted in scarf's block order (IncrementalPCA, one partial fit
+    per chunk, first block last; ann.py:207-256) beside the exact-mode fit the step ran.
+    Reports its wall time and the principal angles between the two loading subspaces."""
+    import numpy as np
+    import torch
+
+    from scarf_amd.pca import fit_pca
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    fit = fit_pca(csr, res.stats, params.norm(), res.loadings.shape[1], params.batch_size,
+                  mode="ipca", precision=params.gram_precision, eig=params.pca_eig)
+    torch.cuda.synchronize()
+    ms = 1e3 * (time.perf_counter() - t0)
+    A, B = res.loadings, fit.loadings
+    cos = np.linalg.svd(A.T @ B, compute_uv=False)
+    per = np.abs(np.sum(A * B, axis=0))  # component by component, after sign alignment
+    lead = int(np.argmax(per < 0.999)) if (per < 0.999).any() else len(per)
+    out = {"pca_ipca_ms": ms,
+           "pca_ipca_partial_fits": len(fit.state.eig_log),
+           "pca_subspace_vs_ipca": {
+               "min_principal_cos": float(cos.min()),
+               "max_principal_angle_deg": float(np.degrees(np.arccos(min(1.0, cos.min())))),
+               "mean_principal_cos": float(cos.mean()),
+               "leading_components_cos_ge_0.999": lead,
+               "component_cos": [round(float(c), 6) for c in per]}}
+    log(f"ipca fit {ms:.1f} ms; subspace {out['pca_subspace_vs_ipca']['min_principal_cos']:.6f}")
+    return out
+
+
+def cpu_oracle_recall(res, k, n, n_queries, seed=7):
+    """CPU leg, untimed: sampled self queries against the FP64 brute force over every
+    reference row (oracle/knn_ref.c, k + 1 then fix_knn_query) on the GPU's embedding."""
+    import numpy as np
+
+    from oracle import knn as oknn
+    rng = np.random.default_rng(seed + 1)
+    q = np.sort(rng.choice(n, min(n_queries, n), replace=False))
+    E = res.embedding.cpu().numpy()
+    t0 = time.perf_counter()
+    ri, rd = oknn.knn_f64(E[q], E, k + 1)
+    fi, fd, _ = oknn.fix_knn_query(ri, rd, q)
+    got = res.indices[q].cpu().numpy()
+    gd = res.distances[q].cpu().numpy().astype(np.float64)
+    rk = float(np.mean([len(set(got[i]) & set(fi[i])) / k for i in range(len(q))]))
+    k11 = min(11, k)
+    r11 = float(np.mean([len(set(got[i, :k11]) & set(fi[i, :k11])) / k11 for i in range(len(q))]))
+    exact = fd[:, -1].astype(np.float32).astype(np.float64)
+    err = np.abs(gd[:, -1] - exact) / np.maximum(exact, 1e-30)
+    log(f"cpu oracle: recall@{k} {rk} on {len(q)} queries ({time.perf_counter() - t0:.1f} s)")
+    return {f"recall_at_{k}_cpu_oracle": rk, "recall_at_11_cpu_oracle": r11,
+            "kth_dist_rel_err_max_cpu_oracle": float(err.max()),
+            "cpu_oracle_queries": int(len(q)),
+            "cpu_oracle": "FP64 brute force over all reference rows (oracle/knn_ref.c), "
+                          "k + 1 neighbours then fix_knn_query, on the GPU's embedding"}
+
+
+def cpu_baseline_leg(csr, wl, n, d, cpu_chunk):
+    """The reference's CPU algorithm timed on the host's cores over a bounded sample of
+    the workload (oracle/baseline.py), at scarf's default threads and on all cores."""
+    import numpy as np
+
+    from oracle.baseline import cpu_baseline
+    chunk = min(wl["chunk"], cpu_chunk)
+    ns = min(n, 3 * chunk)
+    rpn = csr.rowptr[: ns + 1].cpu().numpy()
+    vv = csr.vals[: int(rpn[-1])].cpu().numpy().view(np.uint32)
+    cc = csr.cols[: int(rpn[-1])].cpu().numpy()
+    dense = np.zeros((ns, d), dtype=np.uint32)
+    dense[np.repeat(np.arange(ns), np.diff(rpn)), cc] = vv
+    log(f"cpu baseline on {ns} cells in chunks of {chunk} ...")
+    cb = cpu_baseline(dense, wl["dims"], wl["k"], chunk, 1e4, 10.0, n)
+    ref, allc = cb["reference_defaults"], cb["all_cores"]
+
+    def setting(r):
+        return {"blas_threads": r["blas_threads"], "knn_threads": r["knn_threads"],
+                "linear_stages_cells_per_s": r["linear_stages_cells_per_s"],
+                "with_bruteforce_knn_cells_per_s": r["cells_per_s"],
+                "stage_seconds_at_full_size": {kk: round(v, 2) for kk, v in
+                                               r["est_seconds"].items()},
+                "sample_seconds": {kk: round(v, 3) for kk, v in r["sample_seconds"].items()}}
+
+    return {
+        "value": allc["linear_stages_cells_per_s"], "unit": "cells/s", "cores": cb["cores"],
+        "kind": "port",
+        "value_is": "all host cores, normalise + stats + IncrementalPCA + projection + "
+                    "smoothing only: an upper bound on scarf's CPU rate, since its hnswlib "
+                    "kNN (not importable here) would add to it",
+        "sample": f"first {ns} cells of the same matrix as {ns // chunk} chunks of {chunk} "
+                  f"rows: numpy normalise + FP64 stats, sklearn IncrementalPCA"
+                  f"({wl['dims'] + 1}) with {allc['partial_fits']} partial fits in scarf's "
+                  f"order, z@W, vectorised smoothing, each scaled linearly to {n} cells; "
+                  f"kNN leg: exact BLAS brute force for {allc['knn_queries']} queries x {ns} "
+                  f"references (hnswlib absent), extrapolated quadratically to {n}^2 pairs",
+        "host_cpu_count": cb["cpu_count"],
+        "settings": {"reference_defaults": setting(ref), "all_cores": setting(allc)},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -217,15 +343,27 @@ def main():
     ap.add_argument("--screen", default=None, choices=["bf16", "fp32", "fp8"])
     ap.add_argument("--recall-queries", type=int, default=None)
     ap.add_argument("--pool", type=int, default=None, help="screen candidates per query")
-    ap.add_argument("--compare-screen", default=None, choices=["bf16", "fp32", "fp8"],
-                    help="also time the kNN stage with this screen on the same embedding")
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample", type=int, default=20_000)
+    ap.add_argument("--compare-screen", default="auto", choices=["auto", "none", "bf16", "fp32",
+                                                                 "fp8"],
+                    help="also time the kNN stage with this screen on the same embedding "
+                         "(auto: the plain FP32 screen when the workload screens in BF16/FP8)")
+    ap.add_argument("--no-pca-compare", action="store_true",
+                    help="skip the untimed scarf-order IPCA fit beside an exact-mode PCA")
+    ap.add_argument("--no-cpu-baseline", action="store_true",
+                    help="skip the CPU leg: the baseline timing and the FP64 oracle check of "
+                         "sampled queries")
+    ap.add_argument("--cpu-chunk", type=int, default=20_000,
+                    help="rows per IPCA chunk of the CPU baseline sample (three chunks)")
+    ap.add_argument("--oracle-queries", type=int, default=256,
+                    help="queries checked against the FP64 CPU brute force")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="after the timed steps, copy 1 GiB device to device once (a kernel of "
                          "known traffic that scales hardware counter readings)")
     ap.add_argument("--serving-seconds", type=float, default=12.0)
+    ap.add_argument("--stream-from-host", default="auto", choices=["auto", "yes", "no"],
+                    help="also time one step that streams the counts from pinned host memory "
+                         "in chunk-sized pieces (auto: the 4m workload)")
     ap.add_argument("--aux", action="store_true",
                     help="also time seed partitions, load_graph and CORAL (key 'aux')")
     args = ap.parse_args()
@@ -349,10 +487,20 @@ def main():
         "mapping_p99_ms": None,
     }
 
+    stream_host = args.stream_from_host == "yes" or (args.stream_from_host == "auto"
+                                                     and args.workload == "4m")
+    if stream_host:
+        out["streamed_from_host"] = streamed_step(csr, params, n)
+
+    if wl["pca"] == "exact" and not args.no_pca_compare:
+        out.update(pca_vs_ipca(res, csr, params))
+
     if args.aux:
         out["aux"] = aux_timings(res, csr, params)
 
-    if args.compare_screen:
+    if args.compare_screen == "auto":
+        args.compare_screen = "fp32" if wl["screen"] != "fp32" else "none"
+    if args.compare_screen != "none":
         alt = gknn.pack_refs(res.embedding, args.compare_screen)
         E = res.embedding
         for rep in range(2):
@@ -367,6 +515,9 @@ def main():
         log(f"compare: {out['compare_screen']}")
         del alt
 
+    if not args.no_cpu_baseline:  # CPU leg, part 1: FP64 oracle check of sampled queries
+        out.update(cpu_oracle_recall(res, wl["k"], n, args.oracle_queries))
+
     if not args.no_serving and args.workload == "1m":
         del res
         torch.cuda.empty_cache()
@@ -380,26 +531,9 @@ def main():
                                     "k=11 + mapping scores; '1k' = 1000-cell requests at 20/s",
                           **sv}
 
-    if not args.no_cpu_baseline:
-        from oracle.baseline import cpu_pipeline_sample
-        ns = min(args.cpu_sample, n)
-        rpn = csr.rowptr[: ns + 1].cpu().numpy()
-        vv = csr.vals[: int(rpn[-1])].cpu().numpy().view(np.uint32)
-        cc = csr.cols[: int(rpn[-1])].cpu().numpy()
-        dense = np.zeros((ns, d), dtype=np.uint32)
-        dense[np.repeat(np.arange(ns), np.diff(rpn)), cc] = vv
-        log(f"cpu baseline on {ns} cells ...")
-        cb = cpu_pipeline_sample(dense, wl["dims"], wl["k"], wl["chunk"], 1e4, 10.0, n)
-        out["cpu_baseline"] = {
-            "value": cb["cells_per_s"], "unit": "cells/s", "cores": cb["cores"],
-            "kind": "port",
-            "sample": f"first {ns} cells of the same matrix: numpy normalise + FP64 stats, "
-                      f"sklearn IncrementalPCA({wl['dims'] + 1}) in scarf order, z@W, exact BLAS "
-                      f"kNN for {cb['knn_queries']} queries (hnswlib absent), vectorised "
-                      f"smoothing; each stage scaled to {n} cells (kNN by queries x refs)",
-            "stage_seconds_at_full_size": {kk: round(v, 2) for kk, v in cb["est_seconds"].items()},
-        }
-        out["gpu_vs_cpu"] = value / cb["cells_per_s"]
+    if not args.no_cpu_baseline:  # CPU leg, part 2: the reference algorithm's CPU rate
+        out["cpu_baseline"] = cpu_baseline_leg(csr, wl, n, d, args.cpu_chunk)
+        out["gpu_vs_cpu"] = value / out["cpu_baseline"]["value"]
     print(json.dumps(out), flush=True)
 
 

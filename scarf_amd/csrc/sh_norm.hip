@@ -44,66 +44,80 @@ __global__ void __launch_bounds__(256) k_libsize(const uint32_t *__restrict__ va
 // Per-feature FP64 sums of x and x^2 over a slab of rows, for columns [c0, c0+cb).
 // Slabs never straddle a chunk: slab s covers rows [c*chunk + j*slab_rows, ...) of
 // chunk c = s / spc, j = s % spc, so every chunk of the matrix is done in one launch.
-// Each wave owns a private LDS accumulator; within a row every mapped column is
-// distinct, so the read-modify-write needs no atomics and the order is fixed.
-constexpr int kStatsWaves = 2;
-__global__ void __launch_bounds__(64 * kStatsWaves) k_norm_stats(
+// One wave per slab.  The slab's non-zeros are a contiguous range of the CSR arrays and
+// are streamed flat, 64 * kStatsU at a time with the next batch's loads issued before
+// the current batch is added, so each wave keeps 2 * kStatsU coalesced loads in flight
+// whatever the row lengths.  Each lane finds the row of its entry by walking the slab's
+// row pointers (staged in LDS with the library sizes; entries only move forward).  The
+// wave owns its FP64 accumulators in LDS; entries of one column from different rows
+// meet in one ds_add, whose lanes the LDS applies in a fixed order, so the sums are
+// bitwise reproducible.
+constexpr int kStatsU = 4;
+constexpr int kStatsSlabRows = 384;  // rows per slab (row pointers + library sizes: 6 KB)
+__global__ void __launch_bounds__(64) k_norm_stats(
     const uint32_t *__restrict__ vals, const int32_t *__restrict__ cols,
     const int64_t *__restrict__ rowptr, int64_t n_rows, int64_t chunk_rows, int spc,
     int64_t slab_rows, const int32_t *__restrict__ col_map, int32_t n_in, int32_t d,
     int32_t cb, double sf, int log_t, const double *__restrict__ libsize,
     double *__restrict__ part /* [n_slabs][2][d] */) {
-  extern __shared__ double acc[];  // [kStatsWaves][2][cb]
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  extern __shared__ double acc[];  // [2][cb] then row pointers [slab_rows + 1], libsize
+  const int lane = threadIdx.x;
   const int32_t c0 = blockIdx.y * cb;
   const int32_t cend = min(d, c0 + cb);
   const int32_t ncol = cend - c0;
-  double *my = acc + (size_t)wave * 2 * cb;
-  for (int i = threadIdx.x; i < kStatsWaves * 2 * cb; i += blockDim.x) acc[i] = 0.0;
-  __syncthreads();
+  int64_t *srp = reinterpret_cast<int64_t *>(acc + 2 * cb);
+  double *sls = acc + 2 * cb + (slab_rows + 1);
   const int64_t chunk = blockIdx.x / spc, j_in = blockIdx.x % spc;
   const int64_t c_end = min(n_rows, (chunk + 1) * chunk_rows);
   const int64_t r0 = chunk * chunk_rows + j_in * slab_rows;
-  const int64_t r1 = min(c_end, r0 + slab_rows);
-  // row pointers and library sizes of the wave's next 64 rows come in one load per lane
-  int64_t ra = 0, rb = 0;
-  double rl = 1.0;
-  int ri = 64;
-  for (int64_t r = r0 + wave; r < r1; r += kStatsWaves, ++ri) {
-    if (ri == 64) {
-      const int64_t rr = r + (int64_t)lane * kStatsWaves;
-      if (rr < r1) {
-        ra = rowptr[rr];
-        rb = rowptr[rr + 1];
-        rl = libsize[rr];
-      }
-      ri = 0;
+  // the last chunk can be shorter than spc slabs: its trailing slabs are empty
+  const int64_t nrow = max<int64_t>(0, min(c_end, r0 + slab_rows) - r0);
+  for (int i = lane; i < 2 * cb; i += 64) acc[i] = 0.0;
+  for (int64_t i = lane; i <= nrow; i += 64) srp[i] = nrow > 0 ? rowptr[r0 + i] : 0;
+  for (int64_t i = lane; i < nrow; i += 64) sls[i] = libsize[r0 + i];
+  __syncthreads();
+  const int64_t e0 = srp[0], e1 = srp[nrow];
+  int row = 0;  // slab-relative row of this lane's current entry
+  uint32_t v[kStatsU], vn[kStatsU];
+  int32_t c[kStatsU], cn[kStatsU];
+#pragma unroll
+  for (int u = 0; u < kStatsU; ++u) {
+    const int64_t e = e0 + lane + 64 * u;
+    c[u] = e < e1 ? cols[e] : -1;
+    v[u] = e < e1 ? vals[e] : 0u;
+  }
+  for (int64_t base = e0; base < e1; base += 64 * kStatsU) {
+    const int64_t nb = base + 64 * kStatsU;
+#pragma unroll
+    for (int u = 0; u < kStatsU; ++u) {  // next batch in flight during this one
+      const int64_t e = nb + lane + 64 * u;
+      cn[u] = e < e1 ? cols[e] : -1;
+      vn[u] = e < e1 ? vals[e] : 0u;
     }
-    const int64_t a = __shfl(ra, ri, 64), b = __shfl(rb, ri, 64);
-    const double l = __shfl(rl, ri, 64);
-    for (int64_t t = a + lane; t < b; t += 64) {
-      int f = map_col(col_map, cols[t], n_in);
-      if (f >= c0 && f < cend) {
-        double x = norm_value(vals[t], sf, l, log_t);
-        int j = f - c0;
-        my[j] += x;
-        my[cb + j] += x * x;
+#pragma unroll
+    for (int u = 0; u < kStatsU; ++u) {
+      const int64_t e = base + lane + 64 * u;
+      if (e < e1) {
+        while (srp[row + 1] <= e) ++row;
+        const int f = map_col(col_map, c[u], n_in);
+        if (f >= c0 && f < cend) {
+          const double x = norm_value(v[u], sf, sls[row], log_t);
+          atomicAdd(&acc[f - c0], x);
+          atomicAdd(&acc[cb + f - c0], x * x);
+        }
       }
     }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int u = 0; u < kStatsU; ++u) {
+      c[u] = cn[u];
+      v[u] = vn[u];
+    }
   }
   __syncthreads();
   double *dst = part + (size_t)blockIdx.x * 2 * d;
-  for (int j = threadIdx.x; j < ncol; j += blockDim.x) {
-    double s = 0.0, q = 0.0;
-#pragma unroll
-    for (int w = 0; w < kStatsWaves; ++w) {
-      s += acc[(size_t)w * 2 * cb + j];
-      q += acc[(size_t)w * 2 * cb + cb + j];
-    }
-    dst[c0 + j] = s;
-    dst[d + c0 + j] = q;
+  for (int j = lane; j < ncol; j += 64) {
+    dst[c0 + j] = acc[j];
+    dst[d + c0 + j] = acc[cb + j];
   }
 }
 
@@ -208,70 +222,100 @@ __global__ void __launch_bounds__(256) k_densify(
   }
 }
 
-// Sparse projection: one wave per row; lane j accumulates output dims j and j+64.
-// The row's (feature, z-delta) pairs are read 64 at a time and broadcast with
-// readlane, so each non-zero costs one coalesced load of a W row per lane.
-template <int DPL>
-__global__ void __launch_bounds__(256) k_project(
+// Sparse projection E[r] = c + sum over the row's non-zeros of (z(x) - z(0)) W[f], with
+// W staged in LDS by column tiles.  W (d x D FP32) is larger than LDS, so the D output
+// columns are split into tiles of TW <= 16; a workgroup stages the d x TW tile of W
+// (row stride odd, so the four rows read at once fall in different banks) and projects
+// a block of rows onto it.  The tiles of one row block run on one XCD, dispatched next
+// to each other, so the CSR rows they all read come from that XCD's L2 after the first.
+// One wave per row: a batch of 64 non-zeros is normalised lane-parallel (x in FP32 from
+// log1pf, z-delta, mapped feature) and staged in LDS; the wave's four 16-lane groups then
+// take every fourth entry, lane c of a group adding delta * W[f][c] to its column; the
+// groups' partial sums are added in a fixed order at the end of the row.
+constexpr int kProjWaves = 8;
+constexpr int kProjGroup = 16;
+
+__global__ void __launch_bounds__(64 * kProjWaves) k_project(
     const uint32_t *__restrict__ vals, const int32_t *__restrict__ cols,
     const int64_t *__restrict__ rowptr, int64_t n_rows, const int32_t *__restrict__ col_map,
-    int32_t n_in, double sf, int log_t, const double *__restrict__ libsize,
+    int32_t n_in, int32_t d, float sf, int log_t, const double *__restrict__ libsize,
     const float *__restrict__ mu, const float *__restrict__ inv_sigma, float clip,
     const float *__restrict__ W, int64_t ldw, int32_t D, const float *__restrict__ cvec,
-    float *__restrict__ E, int64_t lde) {
-  const int lane = threadIdx.x & 63;
-  const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (row >= n_rows) return;
-  const int64_t a = rowptr[row], b = rowptr[row + 1];
-  double l;
-  if (libsize) {
-    l = libsize[row];
-  } else {
-    uint64_t s = 0;
-    for (int64_t t = a + lane; t < b; t += 64)
-      if (map_col(col_map, cols[t], n_in) >= 0) s += vals[t];
-    s = wave_sum_u64(s);
-    l = s == 0 ? 1.0 : (double)s;
+    float *__restrict__ E, int64_t lde, int32_t tw, int32_t wstride, int32_t n_tiles,
+    int64_t rows_per_block, int64_t n_row_blocks) {
+  extern __shared__ float wt[];  // [d][wstride], then the waves' entry stages
+  int2 *stage = reinterpret_cast<int2 *>(wt + (size_t)d * wstride);
+  // XCD-aware order: workgroup b runs on XCD b % 8; its tiles of one row block are
+  // consecutive in that XCD's sequence
+  const int64_t b = blockIdx.x;
+  const int64_t g8 = b / 8, x8 = b % 8;
+  const int64_t rblk = (g8 / n_tiles) * 8 + x8;
+  const int tile = (int)(g8 % n_tiles);
+  if (rblk >= n_row_blocks) return;
+  const int32_t col0 = tile * tw;
+  const int32_t ncol = min(tw, D - col0);
+  for (int64_t i = threadIdx.x; i < (int64_t)d * wstride; i += blockDim.x) {
+    const int32_t f = (int32_t)(i / wstride), c = (int32_t)(i % wstride);
+    wt[i] = c < ncol ? W[(int64_t)f * ldw + col0 + c] : 0.f;
   }
-  float acc[DPL];
-#pragma unroll
-  for (int p = 0; p < DPL; ++p) acc[p] = 0.f;
-  for (int64_t t0 = a; t0 < b; t0 += 64) {
-    const int64_t t = t0 + lane;
-    int f = -1;
-    float delta = 0.f;
-    if (t < b) {
-      f = map_col(col_map, cols[t], n_in);
-      if (f >= 0) {
-        float x = (float)norm_value(vals[t], sf, l, log_t);
-        if (clip > 0.f) {
-          float zs = (x - mu[f]) * inv_sigma[f];
-          float z0 = (0.f - mu[f]) * inv_sigma[f];
-          delta = fminf(fmaxf(zs, -clip), clip) - fminf(fmaxf(z0, -clip), clip);
-        } else {
-          delta = x * inv_sigma[f];
+  __syncthreads();
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int grp = lane / kProjGroup;
+  // lanes past the tile's columns read column 0 (their sums are never written)
+  const int c = lane % kProjGroup < ncol ? lane % kProjGroup : 0;
+  int2 *my = stage + w * 64;
+  const int64_t r_begin = rblk * rows_per_block;
+  const int64_t r_end = min(n_rows, r_begin + rows_per_block);
+  for (int64_t row = r_begin + w; row < r_end; row += kProjWaves) {
+    const int64_t a = rowptr[row], bb = rowptr[row + 1];
+    double l;
+    if (libsize) {
+      l = libsize[row];
+    } else {  // library size over the mapped columns (the mapping target's)
+      uint64_t s = 0;
+      for (int64_t t = a + lane; t < bb; t += 64)
+        if (map_col(col_map, cols[t], n_in) >= 0) s += vals[t];
+      s = wave_sum_u64(s);
+      l = s == 0 ? 1.0 : (double)s;
+    }
+    const float scale = sf / (float)l;
+    float acc = 0.f;
+    for (int64_t t0 = a; t0 < bb; t0 += 64) {
+      const int64_t t = t0 + lane;
+      int f = -1;
+      float delta = 0.f;
+      if (t < bb) {
+        f = map_col(col_map, cols[t], n_in);
+        if (f >= 0) {
+          const float v = (float)vals[t] * scale;
+          const float xv = log_t ? log1pf(v) : v;
+          if (clip > 0.f) {
+            const float zs = (xv - mu[f]) * inv_sigma[f];
+            const float z0 = (0.f - mu[f]) * inv_sigma[f];
+            delta = fminf(fmaxf(zs, -clip), clip) - fminf(fmaxf(z0, -clip), clip);
+          } else {
+            delta = xv * inv_sigma[f];
+          }
         }
       }
-    }
-    const int cnt = (int)min<int64_t>(64, b - t0);
-#pragma unroll 8
-    for (int j = 0; j < cnt; ++j) {
-      int fj = __builtin_amdgcn_readlane(f, j);
-      float dj = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(delta), j));
-      if (fj >= 0) {
-        const float *wr = W + (int64_t)fj * ldw;
-#pragma unroll
-        for (int p = 0; p < DPL; ++p) {
-          int c = lane + 64 * p;
-          if (c < D) acc[p] = fmaf(dj, wr[c], acc[p]);
-        }
+      my[lane] = make_int2(f, __float_as_int(delta));
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      const int cnt = (int)min<int64_t>(64, bb - t0);
+#pragma unroll 4
+      for (int j = grp; j < cnt; j += 64 / kProjGroup) {
+        const int2 e = my[j];
+        if (e.x >= 0) acc = fmaf(__int_as_float(e.y), wt[e.x * wstride + c], acc);
       }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
-  }
-#pragma unroll
-  for (int p = 0; p < DPL; ++p) {
-    int c = lane + 64 * p;
-    if (c < D) E[row * lde + c] = acc[p] + cvec[c];
+    // groups 0 + 1 and 2 + 3, then the two pairs
+    acc += __shfl_xor(acc, 16, 64);
+    acc += __shfl_xor(acc, 32, 64);
+    if (lane < ncol) E[row * lde + col0 + lane] = acc + cvec[col0 + lane];
   }
 }
 
@@ -286,10 +330,9 @@ static StatsPlan stats_plan(int64_t n_rows, int64_t chunk_rows) {
   StatsPlan p;
   if (chunk_rows <= 0 || chunk_rows > n_rows) chunk_rows = n_rows > 0 ? n_rows : 1;
   p.n_chunks = cdiv(n_rows > 0 ? n_rows : 1, chunk_rows);
-  // ~256 rows per slab, at most 64 slabs per chunk
-  int64_t spc = cdiv(chunk_rows, 256);
+  // slabs of at most kStatsSlabRows rows: one wave each
+  int64_t spc = cdiv(chunk_rows, kStatsSlabRows);
   if (spc < 1) spc = 1;
-  if (spc > 64) spc = 64;
   p.spc = spc;
   p.slab_rows = cdiv(chunk_rows, spc);
   return p;
@@ -335,12 +378,14 @@ extern "C" int sh_norm_stats(const uint32_t *vals, const int32_t *cols, const in
     SH_HIP(hipMemcpyAsync(libsize_out, libsize_in, sizeof(double) * n_rows,
                           hipMemcpyDeviceToDevice, st));
   }
-  const int32_t cb = d < 2048 ? d : 2048;  // 2 waves x 2 x 2048 doubles = 64 KiB
+  // accumulators of at most 2 x 2048 doubles (32 KiB) + 6 KiB of row data per wave: four
+  // or five waves per CU; wider matrices take column blocks (the CSR is re-read per block)
+  const int32_t cb = d <= 2048 ? d : (int32_t)cdiv(d, cdiv(d, 2048));
   const int n_cb = (int)cdiv(d, cb);
-  const size_t lds = (size_t)kStatsWaves * 2 * cb * sizeof(double);
+  const size_t lds = (2 * (size_t)cb + 2 * (size_t)plan.slab_rows + 1) * sizeof(double);
   double *part = (double *)workspace;
   hipLaunchKernelGGL(k_norm_stats, dim3((unsigned)(plan.n_chunks * plan.spc), n_cb),
-                     dim3(64 * kStatsWaves), lds, st, vals, cols, rowptr, n_rows, chunk,
+                     dim3(64), lds, st, vals, cols, rowptr, n_rows, chunk,
                      (int)plan.spc, plan.slab_rows, col_map, n_cols_in, d, cb, sf, log_transform,
                      ls, part);
   SH_LAUNCH_CHECK("k_norm_stats");
@@ -399,15 +444,23 @@ extern "C" int sh_project(const uint32_t *vals, const int32_t *cols, const int64
   SH_ARG(col_map || d == n_cols_in, "sh_project: col_map NULL requires d == n_cols_in");
   SH_ARG(mu && inv_sigma && W && c && E_out && rowptr, "sh_project: NULL pointer");
   if (n_rows == 0) return SH_OK;
-  dim3 grid(cdiv(n_rows, 4)), block(256);
-  if (D <= 64)
-    hipLaunchKernelGGL(k_project<1>, grid, block, 0, (hipStream_t)stream, vals, cols, rowptr,
-                       n_rows, col_map, n_cols_in, sf, log_transform, libsize, mu, inv_sigma,
-                       clip, W, ldw, D, c, E_out, lde);
-  else
-    hipLaunchKernelGGL(k_project<2>, grid, block, 0, (hipStream_t)stream, vals, cols, rowptr,
-                       n_rows, col_map, n_cols_in, sf, log_transform, libsize, mu, inv_sigma,
-                       clip, W, ldw, D, c, E_out, lde);
+  // column tiles: the d x stride FP32 tile of W within ~120 KiB of LDS, stride odd
+  int32_t stride = (int32_t)((120 * 1024) / (4 * (int64_t)d));
+  if (stride > 17) stride = 17;
+  if (stride % 2 == 0) --stride;
+  SH_ARG(stride >= 1, "sh_project: d=%d too large for the LDS tile of W", d);
+  const int32_t tmax = stride < kProjGroup ? stride : kProjGroup;
+  const int32_t n_tiles = (int32_t)cdiv(D, tmax);
+  const int32_t tw = (int32_t)cdiv(D, n_tiles);
+  int32_t wstride = tw % 2 ? tw : tw + 1;
+  const size_t lds = (size_t)d * wstride * 4 + (size_t)kProjWaves * 64 * sizeof(int2);
+  const int64_t rpb = 2048;
+  const int64_t n_rb = cdiv(n_rows, rpb);
+  const int64_t n_rb8 = cdiv(n_rb, 8) * 8;
+  const dim3 grid((uint32_t)(n_rb8 * n_tiles)), block(64 * kProjWaves);
+  hipLaunchKernelGGL(k_project, grid, block, lds, (hipStream_t)stream, vals, cols, rowptr,
+                     n_rows, col_map, n_cols_in, d, (float)sf, log_transform, libsize, mu,
+                     inv_sigma, clip, W, ldw, D, c, E_out, lde, tw, wstride, n_tiles, rpb, n_rb);
   SH_LAUNCH_CHECK("k_project");
   return SH_OK;
 }

@@ -24,7 +24,7 @@ import numpy as np
 
 from .device import DeviceCSR, col_map_tensor, require_gpu
 from .knn import GpuIndex, pack_refs, search
-from .normalize import FeatureStats, NormParams, feature_stats
+from .normalize import FeatureStats, HostCSR, NormParams, feature_stats, upload_with_stats
 from .pca import PcaFit, Projector, fit_pca
 from .smooth import edges_of, smooth_knn
 
@@ -109,17 +109,20 @@ def build_graph(csr: DeviceCSR, params: GraphParams, feat_idx=None,
                 use_for_pca: Optional[np.ndarray] = None,
                 n_counts: Optional["torch.Tensor"] = None,
                 loadings: Optional[np.ndarray] = None, timer: Optional[StageTimer] = None,
-                stream=None) -> GraphResult:
+                stream=None, stats: Optional[FeatureStats] = None) -> GraphResult:
+    """The whole path on resident counts.  ``stats`` (from :func:`build_graph_streamed`'s
+    upload) skips the statistics pass."""
     require_gpu()
     timer = timer or StageTimer(False)
     p = params.norm()
     n = csr.n_rows
     k = min(params.k, n - 1)
     cm, d = col_map_tensor(csr.n_cols, feat_idx, device=csr.rowptr.device)
-    timer.mark("start")
-    stats = feature_stats(csr, params.batch_size, p, col_map=cm, d=d, n_counts=n_counts,
-                          stream=stream)
-    timer.mark("normalise_stats")
+    if stats is None:
+        timer.mark("start")
+        stats = feature_stats(csr, params.batch_size, p, col_map=cm, d=d, n_counts=n_counts,
+                              stream=stream)
+        timer.mark("normalise_stats")
     mu = stats.mu.cpu().numpy()
     sigma = stats.sigma.cpu().numpy()
     dims = min(params.dims, n, max(1, params.batch_size - 1))
@@ -150,3 +153,19 @@ def build_graph(csr: DeviceCSR, params: GraphParams, feat_idx=None,
     recall = 100.0 * (n - n_mis) / max(n, 1)
     return GraphResult(mu, sigma, fit.loadings, fit, stats, E, idx, dist, w, recall, n_mis,
                        timer.result(), index)
+
+
+def build_graph_streamed(host: HostCSR, params: GraphParams, feat_idx=None,
+                         timer: Optional[StageTimer] = None, stream=None,
+                         copy_stream=None) -> GraphResult:
+    """The whole path from counts in pinned host memory: the CSR streams to HBM in
+    ``params.batch_size``-row chunks while each landed chunk's statistics are computed
+    (``upload_with_stats``), then the rest runs on the resident copy."""
+    require_gpu()
+    timer = timer or StageTimer(False)
+    cm, d = col_map_tensor(host.n_cols, feat_idx)
+    timer.mark("start")
+    csr, stats = upload_with_stats(host, params.batch_size, params.norm(), col_map=cm, d=d,
+                                   copy_stream=copy_stream, stream=stream)
+    timer.mark("upload_stats")
+    return build_graph(csr, params, feat_idx=feat_idx, timer=timer, stream=stream, stats=stats)
