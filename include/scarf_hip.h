@@ -262,6 +262,19 @@ int sh_mapping_score(const int64_t *idx, const float *dist, int64_t nq, int32_t 
                      int log_transform, double *out, void *workspace,
                      size_t workspace_bytes, sh_stream_t stream);
 size_t sh_mapping_score_workspace(int64_t n_ref, int32_t n_groups);
+/* sh_mapping_score without a workspace argument: the accumulators live in a file-scope
+ * device array of the library (64 MiB + 16 KiB per GPU, held until the process ends), so
+ * the call allocates nothing.  No counterpart in the reference: it is for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops).  Limits:
+ * n_ref * n_groups <= 8,388,608 and n_groups <= 4096 (larger calls return SH_EINVAL;
+ * sh_mapping_score_static_max(n_groups) gives the largest n_ref, 0 for a bad n_groups).
+ * All calls on one GPU share the array: calls that overlap on one GPU (other streams or
+ * host threads) race and must be serialised by the caller. */
+int sh_mapping_score_static(const int64_t *idx, const float *dist, int64_t nq, int32_t k,
+                            const int32_t *groups, int32_t n_groups, int64_t n_ref,
+                            int weighted, double fixed_weight, double multiplier,
+                            int log_transform, double *out, sh_stream_t stream);
+size_t sh_mapping_score_static_max(int32_t n_groups);
 
 /* ---- label transfer ----------------------------------------------------------------
  * Per target row of a mapping table (idx i64 [n x k], dist FP64 [n x k], as the

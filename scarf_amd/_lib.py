@@ -66,6 +66,8 @@ _SIGS = {
     "sh_smooth_knn_workspace": (_sz, [_i64, _i64]),
     "sh_mapping_score": (_int, [_p, _p, _i64, _i32, _p, _i32, _i64, _int, _f64, _f64, _int, _p, _p, _sz, _p]),
     "sh_mapping_score_workspace": (_sz, [_i64, _i32]),
+    "sh_mapping_score_static": (_int, [_p, _p, _i64, _i32, _p, _i32, _i64, _int, _f64, _f64, _int, _p, _p]),
+    "sh_mapping_score_static_max": (_sz, [_i32]),
     "sh_target_classes": (_int, [_p, _p, _i64, _i32, _p, _i64, _f64, _p, _p]),
     "sh_kmeans_pp_workspace": (_sz, [_i64, _i32]),
     "sh_kmeans_pp": (_int, [_p, _int, _i64, _i32, _i64, _p, _i32, _i32, _i64, _p, _p, _p, _sz, _p]),

@@ -4,6 +4,8 @@
 // The sums are kept in unsigned 2^-40 fixed point and accumulated with 64-bit
 // integer atomics, which are associative: the result does not depend on launch
 // order or batch composition.
+#include <mutex>
+
 #include "sh_common.h"
 
 namespace sh {
@@ -202,6 +204,64 @@ extern "C" int sh_mapping_score(const int64_t *idx, const float *dist, int64_t n
                      n_ref, k, multiplier, log_transform, out);
   SH_LAUNCH_CHECK("k_ms_final");
   return SH_OK;
+}
+
+// ---- allocation-free variant ----------------------------------------------------------
+// The accumulators of sh_mapping_score live in a file-scope device array, so a caller
+// that cannot allocate (code under graph capture, a tight serving loop) passes only its
+// inputs and outputs.  Each GPU holds its own copy of the array for the life of the
+// process; its device address is looked up once per device with hipGetSymbolAddress and
+// only that address is handed to kernels.  Every call on one GPU shares the array:
+// calls that overlap on one GPU (different streams or host threads) race.
+namespace sh {
+constexpr int64_t kScratchAccWords = (int64_t)1 << 23;  // n_ref * n_groups <= 8,388,608
+constexpr int32_t kScratchMaxGroups = 4096;
+constexpr size_t kScratchBytes = (size_t)kScratchAccWords * sizeof(uint64_t) +
+                                 (size_t)kScratchMaxGroups * sizeof(int32_t);
+static __device__ __align__(256) uint64_t scratch[kScratchBytes / sizeof(uint64_t)];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+}  // namespace sh
+
+extern "C" size_t sh_mapping_score_static_max(int32_t n_groups) {
+  if (n_groups < 1 || n_groups > sh::kScratchMaxGroups) return 0;
+  return (size_t)(sh::kScratchAccWords / n_groups);
+}
+
+extern "C" int sh_mapping_score_static(const int64_t *idx, const float *dist, int64_t nq,
+                                       int32_t k, const int32_t *groups, int32_t n_groups,
+                                       int64_t n_ref, int weighted, double fixed_weight,
+                                       double multiplier, int log_transform, double *out,
+                                       sh_stream_t stream) {
+  using namespace sh;
+  SH_ARG(n_ref >= 1 && n_groups >= 1 && n_groups <= kScratchMaxGroups &&
+             n_ref * (int64_t)n_groups <= kScratchAccWords,
+         "sh_mapping_score_static: n_ref x n_groups = %lld x %d exceeds the static scratch "
+         "(%lld accumulators, at most %d groups); use sh_mapping_score with a workspace",
+         (long long)n_ref, n_groups, (long long)kScratchAccWords, kScratchMaxGroups);
+  int dev = -1;
+  SH_HIP(hipGetDevice(&dev));
+  SH_ARG(dev >= 0 && dev < kMaxDevices, "sh_mapping_score_static: device %d out of range", dev);
+  void *base = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+      void *p = nullptr;
+      SH_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+      g_scratch_ptr[dev] = p;
+    }
+    base = g_scratch_ptr[dev];
+  }
+  // same layout as the workspace: accumulators, then the group counts (256-aligned)
+  const size_t need = sh_mapping_score_workspace(n_ref, n_groups);
+  SH_ARG(need <= kScratchBytes, "sh_mapping_score_static: scratch layout exceeds its size");
+  return sh_mapping_score(idx, dist, nq, k, groups, n_groups, n_ref, weighted, fixed_weight,
+                          multiplier, log_transform, out, base, kScratchBytes, stream);
 }
 
 extern "C" int sh_target_classes(const int64_t *idx, const double *dist, int64_t n, int32_t k,
