@@ -161,7 +161,8 @@ int sh_dense_project(const void *x, int x_dtype, int64_t n, int32_t d, int64_t l
 /* Sparse z-score projection: E[r] = sum_f z_rf W[f] with z = (x - mu) * inv_sigma
  * (clipped to +-clip when clip > 0), evaluated as c + sum over the row's non-zeros of
  * (z(x) - z(0)) * W[f], where c = sum_f z(0)_f W[f] (+ alignment fill, see mapping)
- * is precomputed by the caller.  W is FP32 [d x ldw], E is FP32 [n_rows x lde].
+ * is precomputed by the caller.  W is FP32 [d x ldw] with ldw a multiple of 4 and W
+ * 16-byte aligned (rows are gathered 16 bytes at a time), E is FP32 [n_rows x lde].
  * libsize NULL: computed over the mapped columns (mapping's target normalisation).
  * Replaces: ann.py:129-138 (reducer = transform_z(x).dot(loadings)),
  *           mapping_datastore.py:198-204 (target projection). */

@@ -47,6 +47,28 @@ __device__ __forceinline__ double wave_sum_f64(double v) {
   for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, kWave);
   return v;
 }
+// FP64 sum over the wave with DPP lane moves (no LDS round trips): two quad swaps, two
+// rotations within each 16-lane row, then the four row sums read from lanes 0, 16, 32, 48
+// and added in that fixed order.  Every lane gets the same value; the association differs
+// from wave_sum_f64's butterfly.
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov_f64(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, true);
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double readlane_f64(double v, int lane) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double wave_sum_f64_dpp(double v) {
+  v += dpp_mov_f64<0xB1>(v);   // quad_perm [1, 0, 3, 2]: lane ^ 1
+  v += dpp_mov_f64<0x4E>(v);   // quad_perm [2, 3, 0, 1]: lane ^ 2
+  v += dpp_mov_f64<0x124>(v);  // row_ror:4
+  v += dpp_mov_f64<0x128>(v);  // row_ror:8: each lane holds its row's sum
+  return (readlane_f64(v, 0) + readlane_f64(v, 16)) + (readlane_f64(v, 32) + readlane_f64(v, 48));
+}
 __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
   for (int m = 32; m >= 1; m >>= 1) v += shfl_xor_u64(v, m);
   return v;

@@ -236,6 +236,13 @@ struct ScreenShape {
   static constexpr int QPB = 4 * 32 * QG;  // queries per workgroup
   static constexpr int MINB = 3;
 };
+// Register budget of one instantiation: three workgroups per CU (168 VGPRs) unless the
+// query fragments or the compaction sort need more -- the FP8 screen at K >= 128 and the
+// 1024-key buffers -- which then run two per CU instead of spilling to scratch.
+template <class Op, int K, int CAP>
+constexpr int screen_minb() {
+  return (Op::kScaled && K >= 128) || CAP >= 1024 ? 2 : ScreenShape<Op, K>::MINB;
+}
 
 // Candidate buffer of one (query, slice): CAP keys in two halves of CAP / 2, one per lane
 // half (lanes n and n + 32 both serve query n); each lane appends to its own half with a
@@ -295,7 +302,7 @@ __device__ __forceinline__ void append_quad(const floatx16 &acc, float tau, uint
 }
 
 template <class Op, int K, int CAP>
-__global__ void __launch_bounds__(256, (ScreenShape<Op, K>::MINB)) k_screen(
+__global__ void __launch_bounds__(256, (screen_minb<Op, K, CAP>())) k_screen(
     const typename Op::Frag *__restrict__ Qp_, int64_t nq, const unsigned char *__restrict__ Rp,
     int64_t nr, int64_t slice_len, int S, int pool, const int64_t *__restrict__ seeds,
     uint64_t *__restrict__ buf, uint64_t *__restrict__ out, int32_t *__restrict__ out_cnt) {
@@ -976,7 +983,7 @@ extern "C" int sh_knn_pack(const float *E, int64_t n, int32_t D, int64_t lde, in
   return SH_OK;
 }
 
-static int screen_cap(int pool) { return pool <= 64 ? 256 : 512; }
+static int screen_cap(int pool) { return pool <= 64 ? 256 : pool <= 128 ? 512 : 1024; }
 
 extern "C" size_t sh_knn_screen_workspace(int64_t nq, int32_t pool, int32_t n_slices) {
   return (size_t)nq * n_slices * screen_cap(pool) * sizeof(uint64_t);
@@ -1002,7 +1009,9 @@ extern "C" int sh_knn_screen(const void *qpack, int64_t nq, const void *rpack, i
   SH_ARG(nq >= 0 && nr > 0 && D > 0, "sh_knn_screen: bad sizes nq=%lld nr=%lld D=%d",
          (long long)nq, (long long)nr, D);
   SH_ARG(nr < 0xffffffffll, "sh_knn_screen: more than 2^32-1 references");
-  SH_ARG(pool >= 1 && pool <= 128, "sh_knn_screen: pool=%d outside [1, 128]", pool);
+  SH_ARG(pool >= 1 && pool <= 256, "sh_knn_screen: pool=%d outside [1, 256]", pool);
+  SH_ARG(pool <= 128 || dtype == SH_FP8, "sh_knn_screen: pools above 128 are built for the FP8 "
+         "screen only (pool=%d)", pool);
   SH_ARG(n_slices >= 1 && n_slices <= 4096, "sh_knn_screen: n_slices=%d", n_slices);
   SH_ARG(qpack && rpack && cand_keys && cand_cnt, "sh_knn_screen: NULL pointer");
   const size_t need = sh_knn_screen_workspace(nq, pool, n_slices);
@@ -1020,6 +1029,14 @@ extern "C" int sh_knn_screen(const void *qpack, int64_t nq, const void *rpack, i
   hipStream_t st = (hipStream_t)stream;
   uint64_t *buf = (uint64_t *)workspace;
   const bool big = pool > 64;
+  const bool huge = pool > 128;
+#define SH_SCREEN_CASE_1024(OP, KV)                                                          \
+  if (K == KV && huge) {                                                                     \
+    launch_screen<OP, KV, 1024>(st, qpack, nq, rpack, nr, slice_len, S, pool, seed_refs,     \
+                                buf, cand_keys, cand_cnt);                                   \
+    SH_LAUNCH_CHECK("k_screen");                                                             \
+    return SH_OK;                                                                            \
+  }
 #define SH_SCREEN_CASE(OP, KV)                                                               \
   if (K == KV) {                                                                             \
     if (big)                                                                                 \
@@ -1041,6 +1058,8 @@ extern "C" int sh_knn_screen(const void *qpack, int64_t nq, const void *rpack, i
     SH_SCREEN_CASE(OpBF16, 112)
     SH_SCREEN_CASE(OpBF16, 128)
   } else if (dtype == SH_FP8) {
+    SH_SCREEN_CASE_1024(OpFP8, 64)
+    SH_SCREEN_CASE_1024(OpFP8, 128)
     SH_SCREEN_CASE(OpFP8, 64)
     SH_SCREEN_CASE(OpFP8, 128)
   } else if (dtype == SH_F32) {
@@ -1063,6 +1082,7 @@ extern "C" int sh_knn_screen(const void *qpack, int64_t nq, const void *rpack, i
     SH_SCREEN_CASE(OpF32, 136)
   }
 #undef SH_SCREEN_CASE
+#undef SH_SCREEN_CASE_1024
   set_error("sh_knn_screen: dtype %d with packed K=%d not supported", dtype, K);
   return SH_EUNSUPPORTED;
 }
@@ -1075,7 +1095,7 @@ extern "C" int sh_knn_rerank(const float *Q, int64_t nq, int64_t ldq, const floa
                              int64_t *idx_out, float *dist_out, int32_t *flags_out,
                              sh_stream_t stream) {
   SH_ARG(nq >= 0 && nr > 0 && D > 0 && D <= 256, "sh_knn_rerank: bad sizes");
-  SH_ARG(pool >= 1 && pool <= 128 && n_slices >= 1, "sh_knn_rerank: bad pool/slices");
+  SH_ARG(pool >= 1 && pool <= 256 && n_slices >= 1, "sh_knn_rerank: bad pool/slices");
   const int kk = k + ((self_offset >= 0 || self_idx) ? 1 : 0);
   SH_ARG(k >= 1 && kk <= pool && kk <= 127, "sh_knn_rerank: k=%d (+self) must be <= pool=%d", k,
          pool);
@@ -1086,8 +1106,12 @@ extern "C" int sh_knn_rerank(const float *Q, int64_t nq, int64_t ldq, const floa
     hipLaunchKernelGGL(k_rerank<1>, grid, block, 0, (hipStream_t)stream, Q, nq, ldq, R, nr, ldr,
                        D, cand_keys, cand_cnt, pool, n_slices, k, self_offset, self_idx, err_coef,
                        r_norm_max, idx_out, dist_out, flags_out);
-  else
+  else if (pool <= 128)
     hipLaunchKernelGGL(k_rerank<2>, grid, block, 0, (hipStream_t)stream, Q, nq, ldq, R, nr, ldr,
+                       D, cand_keys, cand_cnt, pool, n_slices, k, self_offset, self_idx, err_coef,
+                       r_norm_max, idx_out, dist_out, flags_out);
+  else
+    hipLaunchKernelGGL(k_rerank<4>, grid, block, 0, (hipStream_t)stream, Q, nq, ldq, R, nr, ldr,
                        D, cand_keys, cand_cnt, pool, n_slices, k, self_offset, self_idx, err_coef,
                        r_norm_max, idx_out, dist_out, flags_out);
   SH_LAUNCH_CHECK("k_rerank");

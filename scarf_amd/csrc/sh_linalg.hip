@@ -121,10 +121,10 @@ __global__ void k_merge_matrix(const double *__restrict__ G, int32_t d, double n
 
 // ---- block Lanczos pieces -------------------------------------------------------------
 // W[q][i] = sum_j M[i][j] V[q][j]: one wave per row of M (read once, coalesced), 16 rows
-// per workgroup.  The block V (8 x d) passes through LDS 256 columns at a time, so each
-// workgroup reads it once instead of once per row; 16 waves with 4 row loads each in
-// flight stream M.
-constexpr int kSymWaves = 16, kSymCols = 256;
+// per workgroup.  The block V (8 x d) passes through LDS 1024 columns at a time (64 KiB),
+// so each workgroup reads it once instead of once per row; each lane has its 16 loads of
+// the row's chunk in flight at once.
+constexpr int kSymWaves = 16, kSymCols = 1024;
 __global__ void __launch_bounds__(kSymWaves * 64) k_symm_rows(const double *__restrict__ M,
                                                               int32_t d,
                                                               const double *__restrict__ V,
@@ -157,7 +157,7 @@ __global__ void __launch_bounds__(kSymWaves * 64) k_symm_rows(const double *__re
   }
 #pragma unroll
   for (int q = 0; q < kLP; ++q) {
-    const double sw = wave_sum_f64(acc[q]);
+    const double sw = wave_sum_f64_dpp(acc[q]);
     if (lane == 0 && row_ok) W[(int64_t)q * d + i] = sw;
   }
 }
@@ -185,7 +185,7 @@ __global__ void __launch_bounds__(256) k_cgs_proj(const double *__restrict__ V, 
   }
 #pragma unroll
   for (int q = 0; q < kLP; ++q) {
-    const double sw = wave_sum_f64(acc[q]);
+    const double sw = wave_sum_f64_dpp(acc[q]);
     if (lane == 0) red[w][q] = sw;
   }
   __syncthreads();
@@ -359,7 +359,7 @@ __global__ void __launch_bounds__(kQrThreads) k_cholqr2(const double *__restrict
     }
 #pragma unroll
     for (int e = 0; e < NE; ++e) {
-      const double sw = wave_sum_f64(acc[e]);
+      const double sw = wave_sum_f64_dpp(acc[e]);
       if (lane == 0) red[w][e] = sw;
     }
     __syncthreads();
@@ -429,7 +429,7 @@ __global__ void __launch_bounds__(kQrThreads) k_cholqr2_wide(const double *__res
     }
 #pragma unroll
     for (int e = 0; e < NE; ++e) {
-      const double sw = wave_sum_f64(acc[e]);
+      const double sw = wave_sum_f64_dpp(acc[e]);
       if (lane == 0) red[w][e] = sw;
     }
     __syncthreads();

@@ -222,100 +222,108 @@ __global__ void __launch_bounds__(256) k_densify(
   }
 }
 
-// Sparse projection E[r] = c + sum over the row's non-zeros of (z(x) - z(0)) W[f], with
-// W staged in LDS by column tiles.  W (d x D FP32) is larger than LDS, so the D output
-// columns are split into tiles of TW <= 16; a workgroup stages the d x TW tile of W
-// (row stride odd, so the four rows read at once fall in different banks) and projects
-// a block of rows onto it.  The tiles of one row block run on one XCD, dispatched next
-// to each other, so the CSR rows they all read come from that XCD's L2 after the first.
-// One wave per row: a batch of 64 non-zeros is normalised lane-parallel (x in FP32 from
-// log1pf, z-delta, mapped feature) and staged in LDS; the wave's four 16-lane groups then
-// take every fourth entry, lane c of a group adding delta * W[f][c] to its column; the
-// groups' partial sums are added in a fixed order at the end of the row.
-constexpr int kProjWaves = 8;
+// Sparse projection E[r] = c + sum over the row's non-zeros of (z(x) - z(0)) W[f]
+// (c, the projection of an all-zero row, is precomputed by the caller).  One wave per
+// row.  A batch of 64 non-zeros is normalised lane-parallel (x in FP32 from log1pf, the
+// z-delta, the mapped feature); the wave's four 16-lane groups then take every fourth
+// entry, and lane c of a group loads columns 4c..4c+3 of that entry's W row with one
+// 16-byte load (W rows padded to a multiple of 4 floats) and adds delta * W into four
+// FP32 accumulators.  Each group thus gathers a whole W row per load instruction; the
+// groups' sums are added in a fixed order at the end of the row.  DPL = 64-column
+// passes (D <= 64 * DPL).
 constexpr int kProjGroup = 16;
-
-__global__ void __launch_bounds__(64 * kProjWaves) k_project(
+template <int DPL>
+__global__ void __launch_bounds__(256) k_project(
     const uint32_t *__restrict__ vals, const int32_t *__restrict__ cols,
     const int64_t *__restrict__ rowptr, int64_t n_rows, const int32_t *__restrict__ col_map,
-    int32_t n_in, int32_t d, float sf, int log_t, const double *__restrict__ libsize,
+    int32_t n_in, float sf, int log_t, const double *__restrict__ libsize,
     const float *__restrict__ mu, const float *__restrict__ inv_sigma, float clip,
     const float *__restrict__ W, int64_t ldw, int32_t D, const float *__restrict__ cvec,
-    float *__restrict__ E, int64_t lde, int32_t tw, int32_t wstride, int32_t n_tiles,
-    int64_t rows_per_block, int64_t n_row_blocks) {
-  extern __shared__ float wt[];  // [d][wstride], then the waves' entry stages
-  int2 *stage = reinterpret_cast<int2 *>(wt + (size_t)d * wstride);
-  // XCD-aware order: workgroup b runs on XCD b % 8; its tiles of one row block are
-  // consecutive in that XCD's sequence
-  const int64_t b = blockIdx.x;
-  const int64_t g8 = b / 8, x8 = b % 8;
-  const int64_t rblk = (g8 / n_tiles) * 8 + x8;
-  const int tile = (int)(g8 % n_tiles);
-  if (rblk >= n_row_blocks) return;
-  const int32_t col0 = tile * tw;
-  const int32_t ncol = min(tw, D - col0);
-  for (int64_t i = threadIdx.x; i < (int64_t)d * wstride; i += blockDim.x) {
-    const int32_t f = (int32_t)(i / wstride), c = (int32_t)(i % wstride);
-    wt[i] = c < ncol ? W[(int64_t)f * ldw + col0 + c] : 0.f;
+    float *__restrict__ E, int64_t lde) {
+  const int lane = threadIdx.x & 63;
+  const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (row >= n_rows) return;
+  const int grp = lane / kProjGroup, c4 = lane % kProjGroup;
+  const int64_t a = rowptr[row], b = rowptr[row + 1];
+  double l;
+  if (libsize) {
+    l = libsize[row];
+  } else {  // library size over the mapped columns (the mapping target's)
+    uint64_t s = 0;
+    for (int64_t t = a + lane; t < b; t += 64)
+      if (map_col(col_map, cols[t], n_in) >= 0) s += vals[t];
+    s = wave_sum_u64(s);
+    l = s == 0 ? 1.0 : (double)s;
   }
-  __syncthreads();
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int grp = lane / kProjGroup;
-  // lanes past the tile's columns read column 0 (their sums are never written)
-  const int c = lane % kProjGroup < ncol ? lane % kProjGroup : 0;
-  int2 *my = stage + w * 64;
-  const int64_t r_begin = rblk * rows_per_block;
-  const int64_t r_end = min(n_rows, r_begin + rows_per_block);
-  for (int64_t row = r_begin + w; row < r_end; row += kProjWaves) {
-    const int64_t a = rowptr[row], bb = rowptr[row + 1];
-    double l;
-    if (libsize) {
-      l = libsize[row];
-    } else {  // library size over the mapped columns (the mapping target's)
-      uint64_t s = 0;
-      for (int64_t t = a + lane; t < bb; t += 64)
-        if (map_col(col_map, cols[t], n_in) >= 0) s += vals[t];
-      s = wave_sum_u64(s);
-      l = s == 0 ? 1.0 : (double)s;
+  const float scale = sf / (float)l;
+  const int64_t ld4 = ldw / 4;
+  const float4 *W4 = reinterpret_cast<const float4 *>(W);
+  bool col_ok[DPL];
+#pragma unroll
+  for (int p = 0; p < DPL; ++p) col_ok[p] = 4 * (c4 + kProjGroup * p) < D;
+  float4 acc[DPL];
+#pragma unroll
+  for (int p = 0; p < DPL; ++p) acc[p] = make_float4(0.f, 0.f, 0.f, 0.f);
+  for (int64_t t0 = a; t0 < b; t0 += 64) {
+    const int64_t t = t0 + lane;
+    int f = -1;
+    float delta = 0.f;
+    if (t < b) {
+      f = map_col(col_map, cols[t], n_in);
+      if (f >= 0) {
+        const float v = (float)vals[t] * scale;
+        const float xv = log_t ? log1pf(v) : v;
+        if (clip > 0.f) {
+          const float zs = (xv - mu[f]) * inv_sigma[f];
+          const float z0 = (0.f - mu[f]) * inv_sigma[f];
+          delta = fminf(fmaxf(zs, -clip), clip) - fminf(fmaxf(z0, -clip), clip);
+        } else {
+          delta = xv * inv_sigma[f];
+        }
+      }
     }
-    const float scale = sf / (float)l;
-    float acc = 0.f;
-    for (int64_t t0 = a; t0 < bb; t0 += 64) {
-      const int64_t t = t0 + lane;
-      int f = -1;
-      float delta = 0.f;
-      if (t < bb) {
-        f = map_col(col_map, cols[t], n_in);
-        if (f >= 0) {
-          const float v = (float)vals[t] * scale;
-          const float xv = log_t ? log1pf(v) : v;
-          if (clip > 0.f) {
-            const float zs = (xv - mu[f]) * inv_sigma[f];
-            const float z0 = (0.f - mu[f]) * inv_sigma[f];
-            delta = fminf(fmaxf(zs, -clip), clip) - fminf(fmaxf(z0, -clip), clip);
-          } else {
-            delta = xv * inv_sigma[f];
+    // The trip count is the same for the whole wave, so every lane takes part in every
+    // shuffle (a shuffle from a lane that has left the loop reads 0); lanes past the row's
+    // end hold f = -1 and add nothing.
+    const int cnt = (int)min<int64_t>(64, b - t0);
+#pragma unroll 4
+    for (int j0 = 0; j0 < cnt; j0 += 64 / kProjGroup) {
+      const int j = j0 + grp;
+      const int fj = __shfl(f, j, 64);
+      const float dj = __shfl(delta, j, 64);
+      if (fj >= 0) {
+#pragma unroll
+        for (int p = 0; p < DPL; ++p) {
+          if (col_ok[p]) {
+            const float4 w = W4[(int64_t)fj * ld4 + c4 + kProjGroup * p];
+            acc[p].x = fmaf(dj, w.x, acc[p].x);
+            acc[p].y = fmaf(dj, w.y, acc[p].y);
+            acc[p].z = fmaf(dj, w.z, acc[p].z);
+            acc[p].w = fmaf(dj, w.w, acc[p].w);
           }
         }
       }
-      my[lane] = make_int2(f, __float_as_int(delta));
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      const int cnt = (int)min<int64_t>(64, bb - t0);
-#pragma unroll 4
-      for (int j = grp; j < cnt; j += 64 / kProjGroup) {
-        const int2 e = my[j];
-        if (e.x >= 0) acc = fmaf(__int_as_float(e.y), wt[e.x * wstride + c], acc);
-      }
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
-    // groups 0 + 1 and 2 + 3, then the two pairs
-    acc += __shfl_xor(acc, 16, 64);
-    acc += __shfl_xor(acc, 32, 64);
-    if (lane < ncol) E[row * lde + col0 + lane] = acc + cvec[col0 + lane];
+  }
+  // groups 0 + 1 and 2 + 3, then the two pairs
+#pragma unroll
+  for (int p = 0; p < DPL; ++p) {
+    acc[p].x += __shfl_xor(acc[p].x, 16, 64);
+    acc[p].y += __shfl_xor(acc[p].y, 16, 64);
+    acc[p].z += __shfl_xor(acc[p].z, 16, 64);
+    acc[p].w += __shfl_xor(acc[p].w, 16, 64);
+    acc[p].x += __shfl_xor(acc[p].x, 32, 64);
+    acc[p].y += __shfl_xor(acc[p].y, 32, 64);
+    acc[p].z += __shfl_xor(acc[p].z, 32, 64);
+    acc[p].w += __shfl_xor(acc[p].w, 32, 64);
+    const int col = 4 * (c4 + kProjGroup * p);
+    if (grp == 0 && col_ok[p]) {
+      float *o = E + row * lde + col;
+      o[0] = acc[p].x + cvec[col];
+      if (col + 1 < D) o[1] = acc[p].y + cvec[col + 1];
+      if (col + 2 < D) o[2] = acc[p].z + cvec[col + 2];
+      if (col + 3 < D) o[3] = acc[p].w + cvec[col + 3];
+    }
   }
 }
 
@@ -382,7 +390,8 @@ extern "C" int sh_norm_stats(const uint32_t *vals, const int32_t *cols, const in
   // or five waves per CU; wider matrices take column blocks (the CSR is re-read per block)
   const int32_t cb = d <= 2048 ? d : (int32_t)cdiv(d, cdiv(d, 2048));
   const int n_cb = (int)cdiv(d, cb);
-  const size_t lds = (2 * (size_t)cb + 2 * (size_t)plan.slab_rows + 1) * sizeof(double);
+  const size_t lds =
+      (2 * (size_t)cb + 2 * (size_t)plan.slab_rows + 1) * sizeof(double);
   double *part = (double *)workspace;
   hipLaunchKernelGGL(k_norm_stats, dim3((unsigned)(plan.n_chunks * plan.spc), n_cb),
                      dim3(64), lds, st, vals, cols, rowptr, n_rows, chunk,
@@ -444,23 +453,17 @@ extern "C" int sh_project(const uint32_t *vals, const int32_t *cols, const int64
   SH_ARG(col_map || d == n_cols_in, "sh_project: col_map NULL requires d == n_cols_in");
   SH_ARG(mu && inv_sigma && W && c && E_out && rowptr, "sh_project: NULL pointer");
   if (n_rows == 0) return SH_OK;
-  // column tiles: the d x stride FP32 tile of W within ~120 KiB of LDS, stride odd
-  int32_t stride = (int32_t)((120 * 1024) / (4 * (int64_t)d));
-  if (stride > 17) stride = 17;
-  if (stride % 2 == 0) --stride;
-  SH_ARG(stride >= 1, "sh_project: d=%d too large for the LDS tile of W", d);
-  const int32_t tmax = stride < kProjGroup ? stride : kProjGroup;
-  const int32_t n_tiles = (int32_t)cdiv(D, tmax);
-  const int32_t tw = (int32_t)cdiv(D, n_tiles);
-  int32_t wstride = tw % 2 ? tw : tw + 1;
-  const size_t lds = (size_t)d * wstride * 4 + (size_t)kProjWaves * 64 * sizeof(int2);
-  const int64_t rpb = 2048;
-  const int64_t n_rb = cdiv(n_rows, rpb);
-  const int64_t n_rb8 = cdiv(n_rb, 8) * 8;
-  const dim3 grid((uint32_t)(n_rb8 * n_tiles)), block(64 * kProjWaves);
-  hipLaunchKernelGGL(k_project, grid, block, lds, (hipStream_t)stream, vals, cols, rowptr,
-                     n_rows, col_map, n_cols_in, d, (float)sf, log_transform, libsize, mu,
-                     inv_sigma, clip, W, ldw, D, c, E_out, lde, tw, wstride, n_tiles, rpb, n_rb);
+  SH_ARG(ldw % 4 == 0 && ((uintptr_t)W % 16) == 0,
+         "sh_project: W rows must be 16-byte aligned (ldw a multiple of 4)");
+  dim3 grid(cdiv(n_rows, 4)), block(256);
+  if (D <= 64)
+    hipLaunchKernelGGL(k_project<1>, grid, block, 0, (hipStream_t)stream, vals, cols, rowptr,
+                       n_rows, col_map, n_cols_in, (float)sf, log_transform, libsize, mu,
+                       inv_sigma, clip, W, ldw, D, c, E_out, lde);
+  else
+    hipLaunchKernelGGL(k_project<2>, grid, block, 0, (hipStream_t)stream, vals, cols, rowptr,
+                       n_rows, col_map, n_cols_in, (float)sf, log_transform, libsize, mu,
+                       inv_sigma, clip, W, ldw, D, c, E_out, lde);
   SH_LAUNCH_CHECK("k_project");
   return SH_OK;
 }

@@ -208,8 +208,8 @@ def _search(Q, R, refs, k, pool, self_offset, n_slices, certify, self_idx, strea
     if kk > nr:
         raise ValueError(f"k={k} needs at least {kk} reference rows, index has {nr}")
     pool = max(int(pool), kk)
-    if pool > 128:
-        raise ValueError("pool > 128 is not supported")
+    if pool > 256 or (pool > 128 and refs.dtype != "fp8"):
+        raise ValueError("pool above 128 is supported for the FP8 screen only, up to 256")
     idx = torch.empty((nq, k), dtype=torch.int64, device=dev)
     dist = torch.empty((nq, k), dtype=torch.float32, device=dev)
     flags = torch.zeros(max(nq, 1), dtype=torch.int32, device=dev)

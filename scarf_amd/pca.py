@@ -365,12 +365,16 @@ class Projector:
         c = zc @ L
         if offset is not None:
             c = c + np.asarray(offset, dtype=np.float64)
-        # one host buffer, one copy: [W (d x D) | mu (d) | 1/sigma (d) | c (D)] in FP32
-        host = np.concatenate([L.astype(np.float32).ravel(), mu.astype(np.float32),
+        # one host buffer, one copy: [W (d x Dp) | mu (d) | 1/sigma (d) | c (D)] in FP32,
+        # W rows padded with zeros to Dp = a multiple of 4 floats (16-byte row gathers)
+        Dp = -(-D // 4) * 4
+        Wp = np.zeros((d, Dp), dtype=np.float32)
+        Wp[:, :D] = L
+        host = np.concatenate([Wp.ravel(), mu.astype(np.float32),
                                (1.0 / sigma).astype(np.float32), c.astype(np.float32)])
         dev = torch.from_numpy(host).to(device=device)
-        W = dev[:d * D].view(d, D)
-        o = d * D
+        W = dev[:d * Dp].view(d, Dp)
+        o = d * Dp
         return cls(W, dev[o:o + d], dev[o + d:o + 2 * d], dev[o + 2 * d:o + 2 * d + D],
                    float(clip) if clip else 0.0, D, d)
 
@@ -390,6 +394,6 @@ class Projector:
                   ptr(col_map), csr.n_cols, self.d, float(params.size_factor),
                   int(params.log_transform),
                   ptr(libsize, row_start) if libsize is not None else None, ptr(self.mu),
-                  ptr(self.inv_sigma), self.clip, ptr(self.W), self.D, self.D, ptr(self.c),
-                  ptr(out), out.stride(0), stream_handle(stream))
+                  ptr(self.inv_sigma), self.clip, ptr(self.W), self.W.stride(0), self.D,
+                  ptr(self.c), ptr(out), out.stride(0), stream_handle(stream))
         return out
