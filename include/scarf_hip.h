@@ -144,6 +144,28 @@ int sh_dgemm(int transA, int64_t m, int64_t n, int64_t k, double alpha, const do
              int64_t lda, const double *B, int64_t ldb, double beta, double *C, int64_t ldc,
              sh_stream_t stream);
 
+/* Block subspace iteration with Chebyshev filtering, the default GPU eigensolver of the
+ * same merge matrix (same replaced call as above).  A block of vectors is stored as the
+ * first ncols columns of a d x ldy row-major array (ncols and ldy multiples of 16; ncols
+ * split into equal groups of at most 128 columns, i.e. ncols <= 128 or a multiple of 32
+ * up to 256, ...).
+ * sh_symm_block: one filter step Y2 = alpha M Y1 + beta Y1 + gamma Y0 with M d x d
+ * symmetric (Y0 may be NULL when gamma == 0; Y2 must not alias Y1 or Y0; Y1 16-byte
+ * aligned).  The product is summed over k slices in a fixed order: bit-reproducible.
+ * sh_block_gram: C [na x nb] (row-major, ld nb) = A^T B for column blocks A (d x lda)
+ * and B (d x ldb): the Gram of a block (CholeskyQR) and the projected matrix X^T M X.
+ * sh_block_residual: res[c] = ||XM[:, c] - theta[c] X[:, c]|| for c < b (Ritz residuals). */
+size_t sh_block_gram_workspace(int32_t d, int32_t na, int32_t nb);
+int sh_block_gram(const double *A, int64_t lda, int32_t na, const double *B, int64_t ldb,
+                  int32_t nb, int32_t d, double *C, void *workspace, size_t workspace_bytes,
+                  sh_stream_t stream);
+size_t sh_symm_block_workspace(int32_t d, int32_t ncols, int64_t ldy);
+int sh_symm_block(const double *M, int32_t d, const double *Y1, const double *Y0, int32_t ncols,
+                  int64_t ldy, double alpha, double beta, double gamma, double *Y2,
+                  void *workspace, size_t workspace_bytes, sh_stream_t stream);
+int sh_block_residual(const double *X, const double *XM, int32_t d, int32_t b, int64_t ldy,
+                      const double *theta, double *res, sh_stream_t stream);
+
 /* Dense blocks of normalised values (x FP32 or FP64 [n x ldx], SH_F32 / SH_F64), as
  * scarf's callers hand them to AnnStream.  z = (x - center) / sigma with an IEEE division
  * (center and sigma FP64 [d], both NULL: no scaling), clipped to +-clip when clip > 0.

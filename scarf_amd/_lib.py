@@ -47,6 +47,11 @@ _SIGS = {
     "sh_lanczos_workspace": (_sz, [_i32, _i32]),
     "sh_lanczos_steps": (_int, [_p, _i32, _p, _i32, _i32, _i32, _p, _p, _p, _p, _sz, _p]),
     "sh_dgemm": (_int, [_int, _i64, _i64, _i64, _f64, _p, _i64, _p, _i64, _f64, _p, _i64, _p]),
+    "sh_symm_block_workspace": (_sz, [_i32, _i32, _i64]),
+    "sh_symm_block": (_int, [_p, _i32, _p, _p, _i32, _i64, _f64, _f64, _f64, _p, _p, _sz, _p]),
+    "sh_block_residual": (_int, [_p, _p, _i32, _i32, _i64, _p, _p, _p]),
+    "sh_block_gram_workspace": (_sz, [_i32, _i32, _i32]),
+    "sh_block_gram": (_int, [_p, _i64, _i32, _p, _i64, _i32, _i32, _p, _p, _sz, _p]),
     "sh_dense_z": (_int, [_p, _int, _i64, _i32, _i64, _p, _p, _f64, _p, _i64, _p]),
     "sh_dense_project": (_int, [_p, _int, _i64, _i32, _i64, _p, _p, _f64, _p, _i32, _p, _i64, _p]),
     "sh_project": (_int, [_p, _p, _p, _i64, _p, _i32, _i32, _f64, _int, _p, _p, _p, _f32, _p, _i64, _i32, _p, _p, _i64, _p]),

@@ -462,9 +462,291 @@ __global__ void __launch_bounds__(kQrThreads) k_cholqr2_wide(const double *__res
   }
 }
 
+// ---- block subspace iteration pieces --------------------------------------------------
+// A block of b vectors is stored as the columns of Y (d x ldy row-major, ldy a multiple of
+// 16, columns past b zero).  One Chebyshev filter step is Y2 = alpha M Y1 + beta Y1 +
+// gamma Y0.  The product is split over slices of the k axis so that a d ~ 2000 matrix
+// still fills the GPU: k_symm_block_part computes one 64-row x (16 NT)-column partial per
+// (row tile, k slice, column group) on v_mfma_f64_16x16x4f64, and k_symm_block_sum adds
+// the slices in slice order (bit-reproducible) and applies the three-term update.
+// M is symmetric, so the A operand M[row][k] is read as M[k][row]: each lane quarter
+// reads 16 consecutive doubles of one row of M.  The k-slice of Y1 passes through LDS 32
+// rows at a time, double-buffered, shared by the workgroup's four waves (16 rows each).
+constexpr int kSbRows = 64;
+constexpr int kSbK = 32;
+constexpr int kSbMaxNT = 8;  // 16-column tiles per column group (128 columns)
+
+template <int NT>
+__global__ void __launch_bounds__(256) k_symm_block_part(
+    const double *__restrict__ M, int32_t d, const double *__restrict__ Y, int64_t ldy,
+    int32_t kslice, double *__restrict__ part /* [slices][d][ldy] */) {
+  __shared__ double ys[2][kSbK][16 * NT];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int kr = lane >> 4, cl = lane & 15;
+  const int64_t row = (int64_t)blockIdx.x * kSbRows + 16 * w + cl;  // this lane's A row
+  const int64_t kb = (int64_t)blockIdx.y * kslice;
+  const int64_t ke = min<int64_t>(d, kb + kslice);
+  const int c0 = blockIdx.z * 16 * NT;
+  constexpr int CW = 16 * NT;           // columns of the group
+  constexpr int PAIRS = kSbK * CW / 2;  // double2 per stage
+  auto stage = [&](int buf, int64_t k0) {
+    for (int e = tid; e < PAIRS; e += 256) {
+      const int kk = e / (CW / 2), c2 = 2 * (e % (CW / 2));
+      const int64_t k = k0 + kk;
+      double2 v = make_double2(0.0, 0.0);
+      if (k < ke) v = *reinterpret_cast<const double2 *>(Y + k * ldy + c0 + c2);
+      *reinterpret_cast<double2 *>(&ys[buf][kk][c2]) = v;
+    }
+  };
+  auto a_chunk = [&](int64_t k0, double *a) {
+#pragma unroll
+    for (int s = 0; s < kSbK / 4; ++s) {
+      const int64_t k = k0 + 4 * s + kr;
+      a[s] = (k < ke && row < d) ? M[k * d + row] : 0.0;
+    }
+  };
+  doublex4 acc[NT];
+#pragma unroll
+  for (int j = 0; j < NT; ++j) acc[j] = doublex4{0.0, 0.0, 0.0, 0.0};
+  double a[kSbK / 4], an[kSbK / 4];
+  int buf = 0;
+  if (kb < ke) {
+    stage(0, kb);
+    a_chunk(kb, a);
+  }
+  __syncthreads();
+  for (int64_t k0 = kb; k0 < ke; k0 += kSbK) {
+    const int64_t kn = k0 + kSbK;
+    if (kn < ke) {  // next chunk: LDS stage and A values in flight during this one
+      stage(buf ^ 1, kn);
+      a_chunk(kn, an);
+    }
+#pragma unroll
+    for (int s = 0; s < kSbK / 4; ++s) {
+#pragma unroll
+      for (int j = 0; j < NT; ++j) {
+        const double bv = ys[buf][4 * s + kr][16 * j + cl];
+        acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s], bv, acc[j], 0, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < kSbK / 4; ++s) a[s] = an[s];
+    __syncthreads();
+    buf ^= 1;
+  }
+  // D layout of v_mfma_f64_16x16x4f64: lane l, value i -> row 4 i + l / 16, column l % 16
+  double *P = part + (int64_t)blockIdx.y * d * ldy;
+  const int64_t rbase = (int64_t)blockIdx.x * kSbRows + 16 * w;
+#pragma unroll
+  for (int j = 0; j < NT; ++j)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int64_t r = rbase + 4 * i + kr;
+      if (r < d) P[r * ldy + c0 + 16 * j + cl] = acc[j][i];
+    }
+}
+
+// Y2[r][c] = alpha * sum over slices (in order) + beta Y1[r][c] + gamma Y0[r][c], for the
+// first ncols columns (the others are left untouched).
+__global__ void __launch_bounds__(256) k_symm_block_sum(
+    const double *__restrict__ part, int32_t slices, int32_t d, int64_t ldy, int32_t ncols,
+    double alpha, const double *__restrict__ Y1, double beta, const double *__restrict__ Y0,
+    double gamma, double *__restrict__ Y2) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)d * ncols) return;
+  const int64_t r = e / ncols, c = e % ncols;
+  const int64_t o = r * ldy + c;
+  double s = 0.0;
+  for (int t = 0; t < slices; ++t) s += part[(int64_t)t * d * ldy + o];
+  double v = alpha * s;
+  if (beta != 0.0) v += beta * Y1[o];
+  if (gamma != 0.0) v += gamma * Y0[o];
+  Y2[o] = v;
+}
+
+// Tall-skinny products C[na x nb] = A^T B of two column blocks (A, B: d x lda / ldb
+// row-major): the rows are split into slices, each workgroup computes one 64 x 64 output
+// block of one slice (four waves of 32 x 32, v_mfma_f64_16x16x4f64, operands read
+// straight from memory: each lane quarter reads 16 consecutive doubles of one row), and
+// k_block_gram_sum adds the slices in slice order.
+constexpr int kBgSlices = 128;
+__global__ void __launch_bounds__(256) k_block_gram_part(
+    const double *__restrict__ A, int64_t lda, int32_t na, const double *__restrict__ B,
+    int64_t ldb, int32_t nb, int32_t d, int32_t rows_per_slice,
+    double *__restrict__ part /* [slices][na][nb] */) {
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int kr = lane >> 4, cl = lane & 15;
+  const int bx = (nb + 63) / 64;
+  const int i0 = (int)(blockIdx.y / bx) * 64 + (w >> 1) * 32;  // wave's first output row
+  const int j0 = (int)(blockIdx.y % bx) * 64 + (w & 1) * 32;   // and column
+  const int64_t r0 = (int64_t)blockIdx.x * rows_per_slice;
+  const int64_t r1 = min<int64_t>(d, r0 + rows_per_slice);
+  doublex4 acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int c = 0; c < 2; ++c) acc[a][c] = doublex4{0.0, 0.0, 0.0, 0.0};
+  for (int64_t k0 = r0; k0 < r1; k0 += 4) {
+    const int64_t r = k0 + kr;
+    double av[2], bv[2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+      const int i = i0 + 16 * a + cl;
+      av[a] = (r < r1 && i < na) ? A[r * lda + i] : 0.0;
+    }
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const int j = j0 + 16 * c + cl;
+      bv[c] = (r < r1 && j < nb) ? B[r * ldb + j] : 0.0;
+    }
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int c = 0; c < 2; ++c)
+        acc[a][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[c], acc[a][c], 0, 0, 0);
+  }
+  double *P = part + (int64_t)blockIdx.x * na * nb;
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int i = i0 + 16 * a + 4 * q + kr, j = j0 + 16 * c + cl;
+        if (i < na && j < nb) P[(int64_t)i * nb + j] = acc[a][c][q];
+      }
+}
+
+__global__ void k_block_gram_sum(const double *__restrict__ part, int32_t slices, int32_t n,
+                                 double *__restrict__ C) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  double s = 0.0;
+  for (int t = 0; t < slices; ++t) s += part[(int64_t)t * n + e];
+  C[e] = s;
+}
+
+// res[c] = || XM[:, c] - theta[c] X[:, c] || for c < b: one workgroup per column, the
+// squares summed per thread in row order and then over the threads in a fixed tree.
+__global__ void __launch_bounds__(256) k_block_residual(const double *__restrict__ X,
+                                                        const double *__restrict__ XM,
+                                                        int32_t d, int64_t ldy,
+                                                        const double *__restrict__ theta,
+                                                        double *__restrict__ res) {
+  __shared__ double red[4];
+  const int c = blockIdx.x, tid = threadIdx.x;
+  const double th = theta[c];
+  double s = 0.0;
+  for (int64_t r = tid; r < d; r += 256) {
+    const double v = XM[r * ldy + c] - th * X[r * ldy + c];
+    s += v * v;
+  }
+  s = wave_sum_f64_dpp(s);
+  if ((tid & 63) == 0) red[tid >> 6] = s;
+  __syncthreads();
+  if (tid == 0) res[c] = sqrt((red[0] + red[1]) + (red[2] + red[3]));
+}
+
 }  // namespace sh
 
 using namespace sh;
+
+// k-slice length of sh_symm_block: about 512 workgroups over (row tiles x slices x column
+// groups), slices a multiple of the LDS stage.
+static int64_t symm_block_kslice(int32_t d, int32_t ncols) {
+  const int64_t tiles_r = cdiv(d, kSbRows);
+  const int64_t groups = cdiv(ncols / 16, kSbMaxNT);
+  int64_t slices = cdiv(512, tiles_r * groups);
+  slices = std::max<int64_t>(1, std::min<int64_t>(slices, cdiv(d, kSbK)));
+  return align_up((size_t)cdiv(d, slices), kSbK);
+}
+
+extern "C" size_t sh_symm_block_workspace(int32_t d, int32_t ncols, int64_t ldy) {
+  if (d <= 0 || ncols <= 0 || ldy < ncols) return 0;
+  const int64_t slices = cdiv(d, symm_block_kslice(d, ncols));
+  return align_up((size_t)slices * d * ldy * sizeof(double), 256);
+}
+
+extern "C" int sh_symm_block(const double *M, int32_t d, const double *Y1, const double *Y0,
+                             int32_t ncols, int64_t ldy, double alpha, double beta, double gamma,
+                             double *Y2, void *ws, size_t ws_bytes, sh_stream_t stream) {
+  SH_ARG(M && Y1 && Y2 && ws, "sh_symm_block: NULL pointer");
+  SH_ARG(gamma == 0.0 || Y0, "sh_symm_block: gamma != 0 needs Y0");
+  SH_ARG(d > 0 && ncols > 0 && ncols % 16 == 0 && ldy >= ncols && ldy % 16 == 0,
+         "sh_symm_block: bad sizes d=%d ncols=%d ldy=%lld", d, ncols, (long long)ldy);
+  const int tiles = ncols / 16;
+  const int groups = (int)cdiv(tiles, kSbMaxNT);
+  SH_ARG(tiles % groups == 0 && groups <= 65535,
+         "sh_symm_block: ncols=%d must split into equal groups of at most 128 columns", ncols);
+  SH_ARG(((uintptr_t)Y1 % 16) == 0, "sh_symm_block: Y1 must be 16-byte aligned");
+  SH_ARG(Y2 != Y1 && (Y0 == nullptr || Y2 != Y0), "sh_symm_block: Y2 must not alias Y1 or Y0");
+  SH_ARG(ws_bytes >= sh_symm_block_workspace(d, ncols, ldy), "sh_symm_block: workspace too small");
+  const int nt = tiles / groups;
+  const int64_t kslice = symm_block_kslice(d, ncols);
+  const int64_t slices = cdiv(d, kslice);
+  hipStream_t st = (hipStream_t)stream;
+  double *part = (double *)ws;
+  dim3 grid((uint32_t)cdiv(d, kSbRows), (uint32_t)slices, (uint32_t)groups);
+#define SH_SB_CASE(N)                                                                      \
+  case N:                                                                                  \
+    hipLaunchKernelGGL(k_symm_block_part<N>, grid, dim3(256), 0, st, M, d, Y1, ldy,        \
+                       (int32_t)kslice, part);                                             \
+    break;
+  switch (nt) {
+    SH_SB_CASE(1) SH_SB_CASE(2) SH_SB_CASE(3) SH_SB_CASE(4)
+    SH_SB_CASE(5) SH_SB_CASE(6) SH_SB_CASE(7) SH_SB_CASE(8)
+    default:
+      SH_ARG(false, "sh_symm_block: internal tile count %d", nt);
+  }
+#undef SH_SB_CASE
+  const int64_t n = (int64_t)d * ncols;
+  hipLaunchKernelGGL(k_symm_block_sum, dim3((uint32_t)cdiv(n, 256)), dim3(256), 0, st, part,
+                     (int32_t)slices, d, ldy, ncols, alpha, Y1, beta, Y0, gamma, Y2);
+  SH_LAUNCH_CHECK("sh_symm_block");
+  return SH_OK;
+}
+
+static int32_t block_gram_rows(int32_t d) {
+  return (int32_t)align_up((size_t)cdiv(d, kBgSlices), 4);
+}
+
+extern "C" size_t sh_block_gram_workspace(int32_t d, int32_t na, int32_t nb) {
+  if (d <= 0 || na <= 0 || nb <= 0) return 0;
+  const int64_t slices = cdiv(d, block_gram_rows(d));
+  return align_up((size_t)slices * na * nb * sizeof(double), 256);
+}
+
+extern "C" int sh_block_gram(const double *A, int64_t lda, int32_t na, const double *B,
+                             int64_t ldb, int32_t nb, int32_t d, double *C, void *ws,
+                             size_t ws_bytes, sh_stream_t stream) {
+  SH_ARG(A && B && C && ws, "sh_block_gram: NULL pointer");
+  SH_ARG(d > 0 && na > 0 && nb > 0 && lda >= na && ldb >= nb, "sh_block_gram: bad sizes");
+  SH_ARG(ws_bytes >= sh_block_gram_workspace(d, na, nb), "sh_block_gram: workspace too small");
+  const int32_t rps = block_gram_rows(d);
+  const int64_t slices = cdiv(d, rps);
+  const int64_t blocks = cdiv(na, 64) * cdiv(nb, 64);
+  SH_ARG(blocks <= 65535, "sh_block_gram: output too large");
+  hipStream_t st = (hipStream_t)stream;
+  double *part = (double *)ws;
+  hipLaunchKernelGGL(k_block_gram_part, dim3((uint32_t)slices, (uint32_t)blocks), dim3(256), 0,
+                     st, A, lda, na, B, ldb, nb, d, rps, part);
+  const int64_t n = (int64_t)na * nb;
+  hipLaunchKernelGGL(k_block_gram_sum, dim3((uint32_t)cdiv(n, 256)), dim3(256), 0, st, part,
+                     (int32_t)slices, (int32_t)n, C);
+  SH_LAUNCH_CHECK("sh_block_gram");
+  return SH_OK;
+}
+
+extern "C" int sh_block_residual(const double *X, const double *XM, int32_t d, int32_t b,
+                                 int64_t ldy, const double *theta, double *res,
+                                 sh_stream_t stream) {
+  SH_ARG(X && XM && theta && res, "sh_block_residual: NULL pointer");
+  SH_ARG(d > 0 && b > 0 && ldy >= b, "sh_block_residual: bad sizes");
+  hipLaunchKernelGGL(k_block_residual, dim3((uint32_t)b), dim3(256), 0, (hipStream_t)stream,
+                     X, XM, d, ldy, theta, res);
+  SH_LAUNCH_CHECK("k_block_residual");
+  return SH_OK;
+}
 
 extern "C" int sh_dgemm(int transA, int64_t m, int64_t n, int64_t k, double alpha,
                         const double *A, int64_t lda, const double *B, int64_t ldb, double beta,

@@ -23,6 +23,7 @@ import math
 from typing import Optional
 
 import numpy as np
+import scipy.linalg as _sla
 
 from . import _lib
 from .device import ptr, require_gpu, stream_handle, workspace
@@ -209,6 +210,272 @@ def _ritz_and_verify(M, V, S, k, d, n_top, st):
     _lib.call("sh_dgemm", 1, n_top, d, k, 1.0, ptr(S_dev), n_top, ptr(V), d, 0.0, ptr(Vt), d,
               st)
     Y = torch.empty((n_top, d), dtype=torch.float64, device=dev)
+    _lib.call("sh_dgemm", 0, n_top, d, d, 1.0, ptr(Vt), d, ptr(M), d, 0.0, ptr(Y), d, st)
+    Vh = Vt.cpu().numpy()
+    Yh = Y.cpu().numpy()
+    theta = np.einsum("ij,ij->i", Yh, Vh)
+    order = np.argsort(-theta, kind="stable")
+    theta, Vh, Yh = theta[order], Vh[order], Yh[order]
+    scale = max(abs(theta[0]), np.finfo(float).tiny)
+    res = float(np.max(np.linalg.norm(Yh - theta[:, None] * Vh, axis=1))) / scale
+    orth = float(np.max(np.abs(Vh @ Vh.T - np.eye(n_top))))
+    return theta, Vh, res, orth
+
+
+# ---- block subspace iteration with Chebyshev filtering ---------------------------------
+
+def chebyshev_block_size(n_top: int, d: int) -> int:
+    """Active block of the filtered subspace iteration: the wanted pairs plus about 0.85
+    as many guard vectors (at least 16).  Convergence of the last wanted pair is set by
+    its distance to the block's smallest Ritz value, not to its neighbour."""
+    b = n_top + max(16, int(0.85 * n_top + 0.5))
+    return min(b, d)
+
+
+def _block_ld(b: int) -> int:
+    """Columns of the device block arrays: 16-aligned, and a multiple of 32 past 128 so
+    that sh_symm_block splits them into equal groups of at most 128."""
+    return -(-b // 16) * 16 if b <= 128 else -(-b // 32) * 32
+
+
+def chebyshev_subspace(dev, n_top: int, tol: float, dyn: float = 16.0,
+                       lock_factor: float = 0.1, max_degree: int = 80,
+                       max_iter: int = 100):
+    """Host loop of Chebyshev-filtered block subspace iteration over a backend ``dev``
+    providing ``rayleigh_ritz() -> (theta, res)`` (active block, descending),
+    ``lock(c, theta)``, ``filter(m, center, half_width, scale)``, ``orthonormalize()``
+    and ``result(n_active) -> (w, Vt, res, orth)``.
+
+    Each iteration filters the active block with a Chebyshev polynomial that damps
+    [0, cut] (the merge matrix is positive semi-definite; cut is the block's smallest
+    Ritz value) and is scaled to 1 at an upper bound of the active spectrum, then
+    orthonormalises it (CholeskyQR2) and takes its Ritz pairs.  The degree is chosen so
+    that the filtered block's condition number stays below e^dyn.  Leading pairs whose
+    residual is far below the tolerance (so that removing them perturbs the rest by less
+    than lock_factor * tol * theta_max) are locked and deflated out of the working
+    matrix, which shrinks the dynamic range of the next filters: the well separated top
+    components go first, the slow ones near the noise edge converge last.
+    """
+    theta_max = 0.0
+    locked = 0
+    lam_up = None
+    pending = None  # Ritz pairs of the active block still valid after a lock
+    for it in range(1, max_iter + 1):
+        th, res = dev.rayleigh_ritz() if pending is None else pending
+        pending = None
+        need = n_top - locked
+        theta_max = max(theta_max, float(th[0]))
+        thr = tol * theta_max
+        if np.all(res[:need] <= thr):
+            w, Vt, res_v, orth = dev.result(need)
+            if res_v <= 100.0 * tol and orth <= 1e-10:
+                return w, Vt, EighInfo(int(th.shape[0]) + locked, it, res_v, orth)
+        c = 0
+        for cand in range(need, 0, -1):
+            gap = float(th[cand - 1] - th[cand])
+            if gap > 0 and np.all(res[:cand] * th[:cand] / gap <= lock_factor * thr):
+                c = cand
+                break
+        if c == need:  # every wanted pair is final; the check above decides
+            c = 0
+        if c:
+            # the rest of the block is already a Ritz basis orthogonal to the locked
+            # vectors, so deflating them leaves its Ritz pairs as they are
+            dev.lock(c, th[:c])
+            locked += c
+            lam_up = None
+            pending = (th[c:], res[c:])
+            continue
+        # upper end of the active spectrum: the top Ritz value plus its residual, the
+        # largest seen since the last lock (early Ritz values underestimate); until the
+        # top pair has settled the degree stays low so an underestimate cannot blow up
+        # the block's condition number
+        up = float(th[0] + res[0])
+        lam_up = up if lam_up is None else max(lam_up, up)
+        cut = float(th[-1])
+        if not (cut > 0.0 and lam_up > cut):
+            raise EigenNotConverged(f"degenerate filter interval (cut {cut:.3e}, top {lam_up:.3e})")
+        half = 0.5 * cut
+        x_up = (lam_up - half) / half
+        m = int(dyn / math.acosh(max(x_up, 1.0 + 1e-12)))
+        m = max(2, min(max_degree if res[0] <= 1e-3 * th[0] else 4, m))
+        dev.filter(m, half, half, lam_up)
+        dev.orthonormalize()
+    raise EigenNotConverged(f"no convergence in {max_iter} filtered iterations")
+
+
+def top_eigh_chebyshev(M: "torch.Tensor", n_top: int, tol: float = 1e-12, seed: int = 0,
+                       stream=None, block: Optional[int] = None,
+                       start: Optional[np.ndarray] = None):
+    """Top ``n_top`` eigenpairs of the symmetric positive semi-definite FP64 device matrix
+    ``M`` (d x d) by Chebyshev-filtered block subspace iteration (see
+    :func:`chebyshev_subspace`).  Every d-sized operation runs on the GPU
+    (``sh_symm_block``, ``sh_block_gram``, ``sh_dgemm``, ``sh_block_residual``); the host
+    solves the b x b Rayleigh-Ritz and Cholesky problems.  ``start`` (host, k x d rows)
+    seeds the first k block vectors, e.g. the previous partial fit's components; the rest
+    of the block starts random.  Returns ``(w, Vt, info)`` like :func:`top_eigh_device`."""
+    require_gpu()
+    d = int(M.shape[0])
+    if M.dtype != torch.float64 or M.shape != (d, d) or not M.is_contiguous():
+        raise ValueError("M must be a contiguous float64 [d, d] tensor")
+    if not 1 <= n_top <= d:
+        raise ValueError(f"n_top={n_top} must be in [1, {d}]")
+    b = chebyshev_block_size(n_top, d) if block is None else int(block)
+    if b <= n_top or b > 256 or b > d:
+        raise EigenNotConverged(f"block {b} cannot hold {n_top} pairs plus guards (d={d})")
+    dev = _DeviceBlock(M, n_top, b, seed, stream, start)
+    return chebyshev_subspace(dev, n_top, tol)
+
+
+class _DeviceBlock:
+    """GPU backend of chebyshev_subspace.  The active block is the first ``nb`` columns
+    of d x ld arrays (the rest zero); locked vectors are kept as rows and deflated out of
+    a working copy of M (Mw -= L^T diag(theta) L, two FP64 GEMMs)."""
+
+    def __init__(self, M, n_top: int, b: int, seed: int, stream, start=None):
+        self.M, self.d, self.nb = M, int(M.shape[0]), b
+        self.ld = _block_ld(b)
+        dv = M.device
+        self.st = stream_handle(stream)
+        self.Mw = M.clone()
+        self.buf = [torch.zeros((self.d, self.ld), dtype=torch.float64, device=dv)
+                    for _ in range(4)]
+        self.X = 0  # index of the block in self.buf
+        self.L = torch.empty((n_top, self.d), dtype=torch.float64, device=dv)
+        self.T = torch.empty((n_top, self.d), dtype=torch.float64, device=dv)
+        self.theta_l = np.zeros(0)
+        self.th_dev = torch.zeros(self.ld, dtype=torch.float64, device=dv)
+        self.res_dev = torch.zeros(self.ld, dtype=torch.float64, device=dv)
+        self.G = torch.empty((self.ld, self.ld), dtype=torch.float64, device=dv)
+        L = _lib.lib()
+        # the k-slice count grows as the block shrinks: size for the narrowest block
+        self.ws_sb, self.nb_sb = workspace("chfsi_sb").get(
+            max(L.sh_symm_block_workspace(self.d, c, self.ld) for c in range(16, self.ld + 1, 16)))
+        self.ws_g, self.nb_g = workspace("chfsi_gram").get(
+            L.sh_block_gram_workspace(self.d, self.ld, self.ld))
+        rng = np.random.default_rng(seed)
+        X0 = rng.standard_normal((self.d, b))
+        if start is not None:
+            k = min(int(start.shape[0]), b - 1)
+            X0[:, :k] = np.asarray(start, dtype=np.float64)[:k].T
+        self.buf[0][:, :b].copy_(torch.from_numpy(X0))
+        self.orthonormalize()
+
+    def _cols(self) -> int:
+        return _block_ld(self.nb)
+
+    def _gram(self, A, B, n):
+        _lib.call("sh_block_gram", ptr(A), self.ld, n, ptr(B), self.ld, n, self.d, ptr(self.G),
+                  ptr(self.ws_g), self.nb_g, self.st)
+        return self.G.view(-1)[:n * n].view(n, n)
+
+    def _times_small(self, A, S: np.ndarray, out):
+        """out[:, :n] = A[:, :k] S for a host matrix S (k x n)."""
+        k, n = S.shape
+        S_dev = torch.from_numpy(np.ascontiguousarray(S)).to(self.M.device)
+        _lib.call("sh_dgemm", 0, self.d, n, k, 1.0, ptr(A), self.ld, ptr(S_dev), n, 0.0,
+                  ptr(out), self.ld, self.st)
+
+    def _spare(self, *used):
+        for i in range(len(self.buf)):
+            if i not in used:
+                return i
+        raise RuntimeError("no spare block buffer")
+
+    def orthonormalize(self):
+        n = self.nb
+        for _ in range(2):
+            X = self.buf[self.X]
+            G = self._gram(X, X, n).cpu().numpy()
+            G = 0.5 * (G + G.T)
+            try:
+                R = np.linalg.cholesky(G).T  # G = R^T R
+            except np.linalg.LinAlgError as exc:
+                raise EigenNotConverged("filtered block lost rank (CholeskyQR)") from exc
+            Rinv, info = _sla.lapack.dtrtri(R, lower=0)
+            if info != 0:
+                raise EigenNotConverged("filtered block lost rank (CholeskyQR)")
+            o = self._spare(self.X)
+            self._times_small(X, Rinv, self.buf[o])
+            self.X = o
+
+    def rayleigh_ritz(self):
+        n, cols = self.nb, self._cols()
+        X = self.buf[self.X]
+        xm = self._spare(self.X)
+        XM = self.buf[xm]
+        _lib.call("sh_symm_block", ptr(self.Mw), self.d, ptr(X), None, cols, self.ld, 1.0, 0.0,
+                  0.0, ptr(XM), ptr(self.ws_sb), self.nb_sb, self.st)
+        H = self._gram(X, XM, n).cpu().numpy()
+        # scipy's LAPACK binding: numpy's eigh is an order of magnitude slower at this
+        # size on a many-threaded host
+        w, Q = _sla.eigh(0.5 * (H + H.T))
+        w, Q = w[::-1].copy(), np.ascontiguousarray(Q[:, ::-1])
+        xo = self._spare(self.X, xm)
+        mo = self._spare(self.X, xm, xo)
+        self._times_small(X, Q, self.buf[xo])
+        self._times_small(XM, Q, self.buf[mo])
+        self.th_dev[:n].copy_(torch.from_numpy(w))
+        _lib.call("sh_block_residual", ptr(self.buf[xo]), ptr(self.buf[mo]), self.d, n, self.ld,
+                  ptr(self.th_dev), ptr(self.res_dev), self.st)
+        self.X = xo
+        return w, self.res_dev[:n].cpu().numpy()
+
+    def lock(self, c: int, theta: np.ndarray):
+        nl = self.theta_l.shape[0]
+        X = self.buf[self.X]
+        L = self.L[nl:nl + c]
+        L.copy_(X[:, :c].t())
+        D = torch.from_numpy(np.diag(np.asarray(theta, dtype=np.float64))).to(self.M.device)
+        T = self.T[:c]
+        _lib.call("sh_dgemm", 0, c, self.d, c, 1.0, ptr(D), c, ptr(L), self.d, 0.0, ptr(T),
+                  self.d, self.st)
+        _lib.call("sh_dgemm", 1, self.d, self.d, c, -1.0, ptr(L), self.d, ptr(T), self.d, 1.0,
+                  ptr(self.Mw), self.d, self.st)
+        self.theta_l = np.concatenate([self.theta_l, np.asarray(theta, dtype=np.float64)])
+        o = self._spare(self.X)
+        Y = self.buf[o]
+        Y.zero_()
+        Y[:, :self.nb - c].copy_(X[:, c:self.nb])
+        self.X = o
+        self.nb -= c
+        for i, t in enumerate(self.buf):  # columns past the active block stay zero
+            if i != o:
+                t[:, self.nb:].zero_()
+
+    def filter(self, m: int, center: float, half: float, scale: float):
+        cols = self._cols()
+        sig = half / (scale - center)
+        tau = 2.0 / sig
+        y0 = self.X
+        y1 = self._spare(y0)
+        _lib.call("sh_symm_block", ptr(self.Mw), self.d, ptr(self.buf[y0]), None, cols, self.ld,
+                  sig / half, -center * sig / half, 0.0, ptr(self.buf[y1]), ptr(self.ws_sb),
+                  self.nb_sb, self.st)
+        for _ in range(2, m + 1):
+            sn = 1.0 / (tau - sig)
+            y2 = self._spare(y0, y1)
+            _lib.call("sh_symm_block", ptr(self.Mw), self.d, ptr(self.buf[y1]),
+                      ptr(self.buf[y0]), cols, self.ld, 2.0 * sn / half, -2.0 * sn * center / half,
+                      -sig * sn, ptr(self.buf[y2]), ptr(self.ws_sb), self.nb_sb, self.st)
+            y0, y1, sig = y1, y2, sn
+        self.X = y1
+
+    def result(self, n_active: int):
+        nl = self.theta_l.shape[0]
+        n_top = nl + n_active
+        Vt = torch.empty((n_top, self.d), dtype=torch.float64, device=self.M.device)
+        if nl:
+            Vt[:nl].copy_(self.L[:nl])
+        Vt[nl:].copy_(self.buf[self.X][:, :n_active].t())
+        return _verify_rows(self.M, Vt, self.d, self.st)
+
+
+def _verify_rows(M, Vt, d, st):
+    """Rayleigh quotients of the rows of Vt (device, n x d) under M, sorted descending,
+    with the largest relative residual and the orthogonality defect (one FP64 GEMM)."""
+    n_top = int(Vt.shape[0])
+    Y = torch.empty((n_top, d), dtype=torch.float64, device=M.device)
     _lib.call("sh_dgemm", 0, n_top, d, d, 1.0, ptr(Vt), d, ptr(M), d, 0.0, ptr(Y), d, st)
     Vh = Vt.cpu().numpy()
     Yh = Y.cpu().numpy()

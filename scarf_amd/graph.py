@@ -50,7 +50,7 @@ class GraphParams:
     bandwidth: float = 1.5
     pca_mode: str = "ipca"  # "ipca" (scarf's block order) or "exact"
     gram_precision: str = "fp64"
-    pca_eig: str = "auto"  # "auto" | "device" (GPU block Lanczos) | "host" (LAPACK)
+    pca_eig: str = "auto"  # "auto" | "device" (GPU Chebyshev) | "lanczos" | "host" (LAPACK)
     screen: str = "bf16"  # "fp32" = certified exact, "bf16" = screened
     pool: int = 64
 

@@ -26,7 +26,7 @@ import numpy as np
 
 from . import _lib
 from .device import DeviceCSR, ptr, require_gpu, stream_handle, workspace
-from .eigen import EigenNotConverged, top_eigh_device
+from .eigen import EigenNotConverged, top_eigh_chebyshev, top_eigh_device
 from .normalize import FeatureStats, NormParams, chunk_bounds
 
 try:
@@ -85,9 +85,10 @@ class IPCAState:
         """One partial_fit from the uncentred Gram and column sums of the chunk's z.
 
         ``G`` is a host FP64 array or a device FP64 tensor (then the merge matrix is
-        formed on the GPU).  ``eig``: "device" (GPU block Lanczos), "host" (LAPACK) or
-        "auto" (device for a device ``G`` with d >= EIG_DEVICE_MIN_D, LAPACK when the
-        device solve does not converge within its basis capacity).
+        formed on the GPU).  ``eig``: "device" (GPU Chebyshev-filtered block subspace
+        iteration), "lanczos" (GPU thick-restart block Lanczos), "host" (LAPACK) or "auto"
+        (device for a device ``G`` with d >= EIG_DEVICE_MIN_D, LAPACK when the device solve
+        does not converge).
         """
         on_dev = torch is not None and isinstance(G, torch.Tensor)
         d = int(G.shape[0])
@@ -96,7 +97,7 @@ class IPCAState:
                              f"sample count {n}")
         if self.n_components > d:
             raise ValueError(f"n_components={self.n_components} invalid for n_features={d}")
-        if eig not in ("auto", "device", "host"):
+        if eig not in ("auto", "device", "lanczos", "host"):
             raise ValueError(f"unknown eigensolver {eig!r}")
         colsum = np.asarray(colsum, dtype=np.float64)
         bm = colsum / n
@@ -166,16 +167,21 @@ class IPCAState:
 
     def _eigensolve(self, M, nc: int, on_dev: bool, eig: str, stream):
         d = int(M.shape[0])
-        use_dev = on_dev and (eig == "device" or (eig == "auto" and d >= EIG_DEVICE_MIN_D))
-        if eig == "device" and not on_dev:
-            raise ValueError("eig='device' needs the Gram on the GPU")
+        use_dev = on_dev and (eig in ("device", "lanczos")
+                              or (eig == "auto" and d >= EIG_DEVICE_MIN_D))
+        if eig in ("device", "lanczos") and not on_dev:
+            raise ValueError(f"eig={eig!r} needs the Gram on the GPU")
         if use_dev:
             try:
-                w, Vt, info = top_eigh_device(M, nc, stream=stream)
+                if eig == "lanczos":
+                    w, Vt, info = top_eigh_device(M, nc, stream=stream)
+                else:  # warm start from the previous partial fit's components
+                    w, Vt, info = top_eigh_chebyshev(M, nc, stream=stream,
+                                                     start=self.components_)
                 self.eig_log.append(("device", info.vectors))
                 return w, Vt
             except EigenNotConverged:
-                if eig == "device":
+                if eig in ("device", "lanczos"):
                     raise
         Mh = M.cpu().numpy() if on_dev else M
         self.eig_log.append(("host", d))

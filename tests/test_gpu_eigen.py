@@ -1,4 +1,5 @@
-"""GPU eigensolver (block Lanczos on the merge matrix) against LAPACK.
+"""GPU eigensolvers of the merge matrix (Chebyshev-filtered block subspace iteration, the
+default, and thick-restart block Lanczos) and their kernels against LAPACK / numpy.
 
 Eigenvalues agree to rounding; eigenvectors agree to within the solver's residual
 tolerance divided by the relative eigenvalue gap (the cases keep that below 1e-7);
@@ -44,17 +45,22 @@ CASES = {
 }
 
 
+def _solver(name):
+    from scarf_amd.eigen import top_eigh_chebyshev, top_eigh_device
+    return {"lanczos": top_eigh_device, "chebyshev": top_eigh_chebyshev}[name]
+
+
+@pytest.mark.parametrize("solver", ["chebyshev", "lanczos"])
 @pytest.mark.parametrize("case", sorted(CASES))
-def test_top_eigh_device_vs_lapack(cuda, case):
+def test_top_eigh_device_vs_lapack(cuda, case, solver):
     import torch
-    from scarf_amd.eigen import top_eigh_device
     M, nt = CASES[case]()
     d = M.shape[0]
-    say("eigen", case, "d", d, "n_top", nt)
+    say("eigen", solver, case, "d", d, "n_top", nt)
     w_ref, V_ref = sla.eigh(M, subset_by_index=[d - nt, d - 1])
     w_ref, V_ref = w_ref[::-1], V_ref[:, ::-1].T
     Md = torch.from_numpy(M).to("cuda")
-    w, Vt, info = top_eigh_device(Md, nt)
+    w, Vt, info = _solver(solver)(Md, nt)
     say("  vectors", info.vectors, "cycles", info.cycles, "res", info.max_residual,
         "orth", info.orthogonality)
     np.testing.assert_allclose(w, w_ref, rtol=1e-12, atol=1e-13 * w_ref[0])
@@ -67,13 +73,65 @@ def test_top_eigh_device_vs_lapack(cuda, case):
     assert info.max_residual <= 1e-10 and info.orthogonality <= 1e-10
 
 
-def test_top_eigh_device_deterministic(cuda):
+@pytest.mark.parametrize("solver", ["chebyshev", "lanczos"])
+def test_top_eigh_device_deterministic(cuda, solver):
     import torch
-    from scarf_amd.eigen import top_eigh_device
     M = torch.from_numpy(_gram(700, 4000, 9)).to("cuda")
-    w1, V1, _ = top_eigh_device(M, 11)
-    w2, V2, _ = top_eigh_device(M, 11)
+    w1, V1, _ = _solver(solver)(M, 11)
+    w2, V2, _ = _solver(solver)(M, 11)
     assert np.array_equal(w1, w2) and np.array_equal(V1, V2)
+
+
+@pytest.mark.parametrize("d,ncols,ldy", [(100, 16, 16), (2000, 96, 96), (2000, 192, 192),
+                                         (3000, 256, 256), (517, 48, 64), (2000, 160, 192)])
+def test_symm_block_step(cuda, d, ncols, ldy):
+    """sh_symm_block: Y2 = alpha M Y1 + beta Y1 + gamma Y0 against numpy, columns past
+    ncols untouched; bit-identical on a repeat."""
+    import torch
+    from scarf_amd import _lib
+    from scarf_amd.device import ptr, stream_handle
+    rng = np.random.default_rng(d + ncols)
+    A = rng.standard_normal((d, d))
+    M = A + A.T
+    Y1 = rng.standard_normal((d, ldy))
+    Y0 = rng.standard_normal((d, ldy))
+    al, be, ga = 0.7, -1.3, 0.25
+    ref = al * (M @ Y1[:, :ncols]) + be * Y1[:, :ncols] + ga * Y0[:, :ncols]
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to("cuda")
+    Md, Y1d, Y0d = t(M), t(Y1), t(Y0)
+    out = []
+    for _ in range(2):
+        Y2d = torch.full((d, ldy), 7.0, dtype=torch.float64, device="cuda")
+        ws = torch.empty(_lib.lib().sh_symm_block_workspace(d, ncols, ldy), dtype=torch.uint8,
+                         device="cuda")
+        _lib.call("sh_symm_block", ptr(Md), d, ptr(Y1d), ptr(Y0d), ncols, ldy, al, be, ga,
+                  ptr(Y2d), ptr(ws), ws.numel(), stream_handle())
+        out.append(Y2d.cpu().numpy())
+    Y2 = out[0]
+    np.testing.assert_allclose(Y2[:, :ncols], ref, rtol=1e-12,
+                               atol=1e-13 * np.sqrt(d) * np.abs(M).max() * 4)
+    assert np.all(Y2[:, ncols:] == 7.0)
+    assert np.array_equal(out[0], out[1])
+
+
+@pytest.mark.parametrize("d,na,nb,lda", [(2000, 94, 94, 96), (3000, 187, 187, 192),
+                                         (33, 5, 7, 16), (1000, 130, 64, 160)])
+def test_block_gram(cuda, d, na, nb, lda):
+    import torch
+    from scarf_amd import _lib
+    from scarf_amd.device import ptr, stream_handle
+    rng = np.random.default_rng(na + nb)
+    A = rng.standard_normal((d, lda))
+    B = rng.standard_normal((d, lda))
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to("cuda")
+    Ad, Bd = t(A), t(B)
+    C = torch.empty((na, nb), dtype=torch.float64, device="cuda")
+    ws = torch.empty(_lib.lib().sh_block_gram_workspace(d, na, nb), dtype=torch.uint8,
+                     device="cuda")
+    _lib.call("sh_block_gram", ptr(Ad), lda, na, ptr(Bd), lda, nb, d, ptr(C), ptr(ws),
+              ws.numel(), stream_handle())
+    np.testing.assert_allclose(C.cpu().numpy(), A[:, :na].T @ B[:, :nb], rtol=1e-12,
+                               atol=1e-13 * np.sqrt(d) * 4)
 
 
 def test_merge_matrix_matches_host(cuda):
