@@ -481,6 +481,7 @@ def main():
         "kth_dist_rel_err_mean": rec["kth_dist_rel_err_mean"],
         "recall_queries": rec["recall_queries"],
         "ann_recall_pct": res.recall,
+        "knn_refined_queries": res.n_refined,
         "stage_ms": {kk: round(v, 3) for kk, v in stage_ms[-1].items()},
         "roofline": roofline,
         "mapping_p50_ms": None,

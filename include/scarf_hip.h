@@ -48,6 +48,8 @@ extern "C" {
 #define SH_FLAG_UNCERTIFIED 1 /* FP32 screen could not prove its top-k exact */
 #define SH_FLAG_SELF_MISSING 2 /* fix_knn_query had to drop the last neighbour */
 #define SH_FLAG_SELF_NOT_FIRST 4 /* column 0 was not the query (a miss, ann.py:34-35) */
+#define SH_FLAG_THIN 8 /* low-precision screen: k-th distance within margin x the pool's
+                          largest screening error of the pool's cut-off (re-screen it) */
 
 typedef void *sh_stream_t;
 
@@ -223,13 +225,17 @@ int sh_knn_screen(const void *qpack, int64_t nq, const void *rpack, int64_t nr,
 size_t sh_knn_screen_workspace(int64_t nq, int32_t pool, int32_t n_slices);
 /* Self queries: query q is reference self_idx[q] (self_idx non-NULL) or
  * self_offset + q (self_offset >= 0); k + 1 neighbours are ranked and the query's
- * own column removed as fix_knn_query does.  Neither: a plain k-neighbour query. */
+ * own column removed as fix_knn_query does.  Neither: a plain k-neighbour query.
+ * err_coef >= 0: the FP32 certificate (SH_FLAG_UNCERTIFIED).  margin > 0: the margin
+ * test of a low-precision screen (SH_FLAG_THIN): the k-th exact distance must lie more
+ * than margin x max over the pool |screened key + ||q||^2 - exact distance| below the
+ * pool's cut-off key + ||q||^2. */
 int sh_knn_rerank(const float *Q, int64_t nq, int64_t ldq, const float *R, int64_t nr,
                   int64_t ldr, int32_t D, const uint64_t *cand_keys,
                   const int32_t *cand_cnt, int32_t pool, int32_t n_slices,
                   int32_t k, int64_t self_offset, const int64_t *self_idx,
-                  double err_coef, float r_norm_max, int64_t *idx_out, float *dist_out,
-                  int32_t *flags_out, sh_stream_t stream);
+                  double err_coef, float r_norm_max, double margin, int64_t *idx_out,
+                  float *dist_out, int32_t *flags_out, sh_stream_t stream);
 /* FP64 brute force for the listed query rows (all rows when q_rows is NULL).  flags_out
  * is read and rewritten: an SH_FLAG_UNCERTIFIED bit already set for a row (by the
  * re-rank whose certificate failed) is kept, the self bits are replaced. */

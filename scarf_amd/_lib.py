@@ -24,6 +24,7 @@ DTYPES = {"fp32": SH_F32, "f32": SH_F32, "bf16": SH_BF16, "fp8": SH_FP8}
 SH_FLAG_UNCERTIFIED = 1
 SH_FLAG_SELF_MISSING = 2
 SH_FLAG_SELF_NOT_FIRST = 4
+SH_FLAG_THIN = 8
 
 _p = ctypes.c_void_p
 _i32 = ctypes.c_int32
@@ -61,7 +62,7 @@ _SIGS = {
     "sh_knn_screen": (_int, [_p, _i64, _p, _i64, _i32, _int, _p, _p, _i32, _i32, _p, _p, _p, _p, _sz,
                             _p]),
     "sh_knn_screen_workspace": (_sz, [_i64, _i32, _i32]),
-    "sh_knn_rerank": (_int, [_p, _i64, _i64, _p, _i64, _i64, _i32, _p, _p, _i32, _i32, _i32, _i64, _p, _f64, _f32, _p, _p, _p, _p]),
+    "sh_knn_rerank": (_int, [_p, _i64, _i64, _p, _i64, _i64, _i32, _p, _p, _i32, _i32, _i32, _i64, _p, _f64, _f32, _f64, _p, _p, _p, _p]),
     "sh_knn_exact": (_int, [_p, _i64, _i64, _p, _i64, _p, _i64, _i64, _i32, _i32, _i64, _p, _p, _p, _p, _p, _p]),
     "sh_fix_knn_query": (_int, [_p, _p, _p, _i64, _i32, _p, _p, _p, _p]),
     "sh_locality_order_workspace": (_sz, [_i64]),

@@ -73,6 +73,10 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
   for (int m = 32; m >= 1; m >>= 1) v += shfl_xor_u64(v, m);
   return v;
 }
+__device__ __forceinline__ double wave_max_f64(double v) {
+  for (int m = 32; m >= 1; m >>= 1) v = fmax(v, __shfl_xor(v, m, kWave));
+  return v;
+}
 __device__ __forceinline__ float wave_max_f32(float v) {
   for (int m = 32; m >= 1; m >>= 1) v = fmaxf(v, __shfl_xor(v, m, kWave));
   return v;

@@ -649,7 +649,7 @@ __global__ void __launch_bounds__(256) k_rerank(
     int64_t nr, int64_t ldr, int32_t D, const uint64_t *__restrict__ cand,
     const int32_t *__restrict__ cand_cnt, int pool, int S, int k, int64_t self_offset,
     const int64_t *__restrict__ self_idx,
-    double err_coef, float r_norm_max, int64_t *__restrict__ idx_out,
+    double err_coef, float r_norm_max, double margin, int64_t *__restrict__ idx_out,
     float *__restrict__ dist_out, int32_t *__restrict__ flags) {
   __shared__ float qv[4][256];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -709,7 +709,11 @@ __global__ void __launch_bounds__(256) k_rerank(
   const uint64_t lastkey = wave_elem_u64<PE>(v, npool > 0 ? npool - 1 : 0);
 
   // 2. exact distances of the pool
+  double qn = 0.0;  // ||q||^2 in FP64 (the certificate and the margin test)
+  if (err_coef >= 0.0 || margin > 0.0)
+    for (int t = 0; t < D; ++t) qn += (double)ql[t] * (double)ql[t];
   KeyIdx c[PE];
+  double emax = 0.0;  // largest screening error |key + ||q||^2 - exact| over the pool
 #pragma unroll
   for (int i = 0; i < PE; ++i) {
     const int e = lane * PE + i;
@@ -717,11 +721,14 @@ __global__ void __launch_bounds__(256) k_rerank(
       const uint32_t j = (uint32_t)(v[i] & 0xffffffffu);
       c[i].d = sqdist_exact(ql, R + (int64_t)j * ldr, D);
       c[i].i = j;
+      if (margin > 0.0)
+        emax = fmax(emax, fabs((double)ord_float((uint32_t)(v[i] >> 32)) + qn - c[i].d));
     } else {
       c[i].d = INFINITY;
       c[i].i = 0xffffffffu;
     }
   }
+  if (margin > 0.0) emax = wave_max_f64(emax);
   wave_bitonic_sort<PE>(c);
 
   const bool self = self_idx != nullptr || self_offset >= 0;
@@ -736,8 +743,6 @@ __global__ void __launch_bounds__(256) k_rerank(
   if (err_coef >= 0.0) {
     bool certified = true;
     if (nr > (int64_t)npool && kk > 0) {
-      double qn = 0.0;
-      for (int t = 0; t < D; ++t) qn += (double)ql[t] * (double)ql[t];
       const double rq = sqrt(qn);
       const double err =
           err_coef * ((double)r_norm_max * r_norm_max + 2.0 * rq * (double)r_norm_max + qn);
@@ -755,6 +760,21 @@ __global__ void __launch_bounds__(256) k_rerank(
       certified = dk < bound;
     }
     if (!certified) flag |= SH_FLAG_UNCERTIFIED;
+  }
+  // margin test of a low-precision screen: a reference left out of the pool has a
+  // screened key >= lastkey, so a true neighbour was lost only if its screening error
+  // reached gap = lastkey + ||q||^2 - d_kk.  The pool shows the errors the screen made
+  // near this query; rows whose gap is under margin times the largest of them are
+  // flagged for a wider second screen.
+  if (margin > 0.0 && nr > (int64_t)npool && kk > 0) {
+    const int e = (kk <= npool ? kk : npool) - 1;
+    double mine = 0.0;
+#pragma unroll
+    for (int i = 0; i < PE; ++i)
+      if (lane * PE + i == e) mine = c[i].d;
+    const double dk = __shfl(mine, e / PE, kWave);
+    const double gap = (double)ord_float((uint32_t)(lastkey >> 32)) + qn - dk;
+    if (gap <= margin * emax) flag |= SH_FLAG_THIN;
   }
   // gather the top kk (index, distance) into lane-indexed registers: lane t <- elem t
   double dd[2];
@@ -1091,7 +1111,7 @@ extern "C" int sh_knn_rerank(const float *Q, int64_t nq, int64_t ldq, const floa
                              int64_t ldr, int32_t D, const uint64_t *cand_keys,
                              const int32_t *cand_cnt, int32_t pool, int32_t n_slices, int32_t k,
                              int64_t self_offset, const int64_t *self_idx, double err_coef,
-                             float r_norm_max,
+                             float r_norm_max, double margin,
                              int64_t *idx_out, float *dist_out, int32_t *flags_out,
                              sh_stream_t stream) {
   SH_ARG(nq >= 0 && nr > 0 && D > 0 && D <= 256, "sh_knn_rerank: bad sizes");
@@ -1105,15 +1125,15 @@ extern "C" int sh_knn_rerank(const float *Q, int64_t nq, int64_t ldq, const floa
   if (pool <= 64)
     hipLaunchKernelGGL(k_rerank<1>, grid, block, 0, (hipStream_t)stream, Q, nq, ldq, R, nr, ldr,
                        D, cand_keys, cand_cnt, pool, n_slices, k, self_offset, self_idx, err_coef,
-                       r_norm_max, idx_out, dist_out, flags_out);
+                       r_norm_max, margin, idx_out, dist_out, flags_out);
   else if (pool <= 128)
     hipLaunchKernelGGL(k_rerank<2>, grid, block, 0, (hipStream_t)stream, Q, nq, ldq, R, nr, ldr,
                        D, cand_keys, cand_cnt, pool, n_slices, k, self_offset, self_idx, err_coef,
-                       r_norm_max, idx_out, dist_out, flags_out);
+                       r_norm_max, margin, idx_out, dist_out, flags_out);
   else
     hipLaunchKernelGGL(k_rerank<4>, grid, block, 0, (hipStream_t)stream, Q, nq, ldq, R, nr, ldr,
                        D, cand_keys, cand_cnt, pool, n_slices, k, self_offset, self_idx, err_coef,
-                       r_norm_max, idx_out, dist_out, flags_out);
+                       r_norm_max, margin, idx_out, dist_out, flags_out);
   SH_LAUNCH_CHECK("k_rerank");
   return SH_OK;
 }

@@ -73,6 +73,7 @@ class GraphResult:
     n_mis: int
     timings_ms: Dict[str, float]
     index: GpuIndex
+    n_refined: int = 0  # FP8 screen: queries screened a second time (SH_FLAG_THIN)
 
     def edges(self) -> np.ndarray:
         n, k = self.indices.shape
@@ -150,9 +151,10 @@ def build_graph(csr: DeviceCSR, params: GraphParams, feat_idx=None,
     timer.mark("smooth")
     # fix_knn_query's n_mis: rows whose first neighbour (of k + 1) was not the row itself
     n_mis = int(((flags & 4) > 0).sum().item())
+    n_refined = int(((flags & 8) > 0).sum().item())
     recall = 100.0 * (n - n_mis) / max(n, 1)
     return GraphResult(mu, sigma, fit.loadings, fit, stats, E, idx, dist, w, recall, n_mis,
-                       timer.result(), index)
+                       timer.result(), index, n_refined)
 
 
 def build_graph_streamed(host: HostCSR, params: GraphParams, feat_idx=None,

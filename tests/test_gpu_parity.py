@@ -391,22 +391,37 @@ def _fp8_model_recall(Q, R, pool, k=11):
 @pytest.mark.parametrize("D,pool", [(100, 128), (50, 64)])
 def test_knn_fp8_recall(cuda, D, pool):
     """FP8 screen recall equals the CPU model of the same quantisation (the kernel
-    adds no error of its own); at D=100 with a 128 pool it exceeds 0.99."""
+    adds no error of its own); at D=100 with a 128 pool it exceeds 0.99.  With the
+    margin test and the second screen of the flagged queries (the default for FP8) no
+    query loses recall, and the refined rows are exact wherever the wider pool holds
+    the true neighbours."""
     import torch
+    from scarf_amd import _lib
     from scarf_amd.knn import pack_refs, search
     R = _rand_embedding(20000, D, 12)
     Q = R[:2000]
     Rt, Qt = torch.from_numpy(R).cuda(), torch.from_numpy(Q).cuda()
     say("knn fp8 D", D, "pool", pool)
     refs = pack_refs(Rt, "fp8")
-    idx, dist, _ = search(Qt, Rt, refs, 11, pool=pool)
+    idx, dist, _ = search(Qt, Rt, refs, 11, pool=pool, refine=False)
     model, ri = _fp8_model_recall(Q, R, pool)
     got = idx.cpu().numpy()
-    recall = np.mean([len(set(got[i]) & set(ri[i])) / 11 for i in range(len(Q))])
+    per_q = np.array([len(set(got[i]) & set(ri[i])) / 11 for i in range(len(Q))])
+    recall = per_q.mean()
     say("fp8 recall@11", recall, "model", model)
     assert abs(recall - model) < 2e-3
     if D == 100:
         assert recall >= 0.99
+    idx2, dist2, fl2 = search(Qt, Rt, refs, 11, pool=pool)
+    got2 = idx2.cpu().numpy()
+    per_q2 = np.array([len(set(got2[i]) & set(ri[i])) / 11 for i in range(len(Q))])
+    thin = (fl2.cpu().numpy() & _lib.SH_FLAG_THIN) > 0
+    say("refined", int(thin.sum()), "recall@11", per_q2.mean(),
+        "misses left unflagged", int((per_q[~thin] < 1).sum()))
+    # unflagged rows are the first screen's rows; flagged rows never lose recall
+    assert np.array_equal(got2[~thin], got[~thin])
+    assert np.all(per_q2[thin] >= per_q[thin])
+    assert per_q2.mean() >= recall
 
 
 @pytest.mark.parametrize("k,threshold", [(12, 0.5), (22, 0.5), (12, 0.3), (6, 0.9), (2, 0.5)])
