@@ -141,7 +141,7 @@ def thick_restart_lanczos(dev, n_top: int, keep: int, kin: int, tol: float,
             H[m:m + P, :m] = b
             H[:m, m:m + P] = b.T
         H[m:, m:] = lanczos_matrix(td, ts, J - J0)
-        w_all, Z = np.linalg.eigh(H)
+        w_all, Z = _sla.eigh(H)
         mk = min(keep, k)
         theta = w_all[::-1][:mk].copy()
         S = np.ascontiguousarray(Z[:, ::-1][:, :mk])
@@ -327,6 +327,19 @@ def top_eigh_chebyshev(M: "torch.Tensor", n_top: int, tol: float = 1e-12, seed: 
     return chebyshev_subspace(dev, n_top, tol)
 
 
+_RANDOM_BLOCKS = {}
+
+
+def _random_block(d: int, b: int, seed: int) -> np.ndarray:
+    """The seeded random start block (d x b), drawn once per shape: every partial fit of
+    a run solves at the same size."""
+    key = (d, b, seed)
+    blk = _RANDOM_BLOCKS.get(key)
+    if blk is None:
+        blk = _RANDOM_BLOCKS[key] = np.random.default_rng(seed).standard_normal((d, b))
+    return blk
+
+
 class _DeviceBlock:
     """GPU backend of chebyshev_subspace.  The active block is the first ``nb`` columns
     of d x ld arrays (the rest zero); locked vectors are kept as rows and deflated out of
@@ -353,9 +366,9 @@ class _DeviceBlock:
             max(L.sh_symm_block_workspace(self.d, c, self.ld) for c in range(16, self.ld + 1, 16)))
         self.ws_g, self.nb_g = workspace("chfsi_gram").get(
             L.sh_block_gram_workspace(self.d, self.ld, self.ld))
-        rng = np.random.default_rng(seed)
-        X0 = rng.standard_normal((self.d, b))
+        X0 = _random_block(self.d, b, seed)
         if start is not None:
+            X0 = X0.copy()
             k = min(int(start.shape[0]), b - 1)
             X0[:, :k] = np.asarray(start, dtype=np.float64)[:k].T
         self.buf[0][:, :b].copy_(torch.from_numpy(X0))
@@ -389,7 +402,7 @@ class _DeviceBlock:
             G = self._gram(X, X, n).cpu().numpy()
             G = 0.5 * (G + G.T)
             try:
-                R = np.linalg.cholesky(G).T  # G = R^T R
+                R = _sla.cholesky(G, lower=False)  # G = R^T R (scipy: see rayleigh_ritz)
             except np.linalg.LinAlgError as exc:
                 raise EigenNotConverged("filtered block lost rank (CholeskyQR)") from exc
             Rinv, info = _sla.lapack.dtrtri(R, lower=0)
@@ -407,8 +420,8 @@ class _DeviceBlock:
         _lib.call("sh_symm_block", ptr(self.Mw), self.d, ptr(X), None, cols, self.ld, 1.0, 0.0,
                   0.0, ptr(XM), ptr(self.ws_sb), self.nb_sb, self.st)
         H = self._gram(X, XM, n).cpu().numpy()
-        # scipy's LAPACK binding: numpy's eigh is an order of magnitude slower at this
-        # size on a many-threaded host
+        # scipy's LAPACK binding: numpy's eigh and cholesky are an order of magnitude
+        # slower at these sizes on a many-threaded host
         w, Q = _sla.eigh(0.5 * (H + H.T))
         w, Q = w[::-1].copy(), np.ascontiguousarray(Q[:, ::-1])
         xo = self._spare(self.X, xm)
