@@ -18,6 +18,15 @@ Scheduling
 * Determinism: every kernel's result for a query row depends only on that row and the
   resident reference (projection per row, canonical kNN pool, 2^-40 fixed-point score
   sums), so a request's answer is bit-identical alone and in any mix.
+* Uploads go through per-stream pinned staging buffers (reused; a stream is handed out
+  again only after its work has completed), so the host-to-device copies are
+  asynchronous and overlap the other streams' kernels.
+* Cancellation: ``future.cancel()`` on a request's future.  A queued request is dropped;
+  an admitted one issues no further chunks and releases its memory and slot as soon as
+  its chunks in flight have completed.
+* Idle waiting: when nothing can be issued the engine sleeps on a condition variable
+  that a submission or a cancellation wakes, checking its in-flight events every
+  ``poll_s`` seconds.
 """
 from __future__ import annotations
 
@@ -84,7 +93,8 @@ class MappingServer:
                  index: GpuIndex, source_feat_ids: Sequence, query_ids: Sequence,
                  norm: NormParams = NormParams(), k: int = 11, mem_budget: int = 4 << 30,
                  chunk_rows: int = 16384, n_streams: int = 4, multiplier: float = 1000.0,
-                 log_transform_scores: bool = True, disable_scaling: bool = False):
+                 log_transform_scores: bool = True, disable_scaling: bool = False,
+                 poll_s: float = 0.0005):
         require_gpu()
         self.index = index
         self.k = int(k)
@@ -102,7 +112,9 @@ class MappingServer:
         self.multiplier = float(multiplier)
         self.log_scores = bool(log_transform_scores)
         self.streams = [torch.cuda.Stream() for _ in range(max(1, n_streams))]
+        self._staging = [None] * len(self.streams)  # pinned upload buffer per stream
         self._free_streams = collections.deque(range(len(self.streams)))
+        self.poll_s = float(poll_s)
         self._queue = collections.deque()
         self._active: List[_Req] = []
         self._inflight = []  # (event, req, kind, stream_id)
@@ -146,6 +158,13 @@ class MappingServer:
             self._lock.notify()
         return fut
 
+    def cancel(self, fut: cf.Future) -> bool:
+        """Cancels a submitted request (the same as ``fut.cancel()``)."""
+        ok = fut.cancel()
+        with self._lock:
+            self._lock.notify()
+        return ok
+
     def start(self):
         self._stop = False
         self._thread = threading.Thread(target=self._engine, name="scarf-amd-mapping",
@@ -174,6 +193,8 @@ class MappingServer:
     # -- engine ---------------------------------------------------------------------------
     def _admit(self):
         now = time.perf_counter()
+        if any(r.fut.cancelled() for r in self._queue):
+            self._queue = collections.deque(r for r in self._queue if not r.fut.cancelled())
         while self._queue and self._used + self._queue[0].need <= self.mem_budget:
             req = self._queue.popleft()
             self._used += req.need
@@ -185,20 +206,38 @@ class MappingServer:
         n = len(self._active)
         for off in range(n):
             req = self._active[(self._rr + off) % n]
-            if req.next_row < req.n:
+            if req.next_row < req.n and not req.fut.cancelled():
                 self._rr = (self._rr + off + 1) % max(n, 1)
                 return req
         return None
 
-    def _upload(self, req: _Req, stream):
+    def _pinned(self, sid: int, nbytes: int):
+        """The stream's pinned staging buffer, grown to at least ``nbytes``."""
+        buf = self._staging[sid]
+        if buf is None or buf.numel() < nbytes:
+            buf = torch.empty(max(nbytes, 1 << 20) * 5 // 4, dtype=torch.uint8, pin_memory=True)
+            self._staging[sid] = buf
+        return buf
+
+    def _upload(self, req: _Req, stream, sid: int):
         c = req.csr
+        parts = [np.ascontiguousarray(c.vals, dtype=np.uint32).view(np.int32),
+                 np.ascontiguousarray(c.cols, dtype=np.int32),
+                 np.ascontiguousarray(c.rowptr, dtype=np.int64)]
+        offs, o = [], 0
+        for a in parts:
+            offs.append(o)
+            o += -(-a.nbytes // 256) * 256
+        stage = self._pinned(sid, o)
+        host = stage.numpy()
+        dev = []
         with torch.cuda.stream(stream):
-            vals = torch.from_numpy(np.ascontiguousarray(c.vals, dtype=np.uint32).view(np.int32))
-            req.dev = DeviceCSR(vals.to("cuda", non_blocking=True),
-                                torch.from_numpy(np.ascontiguousarray(c.cols, np.int32)).to(
-                                    "cuda", non_blocking=True),
-                                torch.from_numpy(np.ascontiguousarray(c.rowptr, np.int64)).to(
-                                    "cuda", non_blocking=True), c.n_cols)
+            for a, off in zip(parts, offs):
+                host[off:off + a.nbytes] = a.view(np.uint8).reshape(-1)
+                src = stage[off:off + a.nbytes].view(torch.int32 if a.dtype == np.int32
+                                                     else torch.int64)
+                dev.append(src.to("cuda", non_blocking=True))
+            req.dev = DeviceCSR(dev[0], dev[1], dev[2], c.n_cols)
             req.idx = torch.empty((req.n, self.k), dtype=torch.int64, device="cuda")
             req.dist = torch.empty((req.n, self.k), dtype=torch.float32, device="cuda")
             req.uploaded = torch.cuda.Event()
@@ -207,7 +246,7 @@ class MappingServer:
     def _issue_chunk(self, req: _Req, sid: int):
         stream = self.streams[sid]
         if req.dev is None:
-            self._upload(req, stream)
+            self._upload(req, stream, sid)
         else:
             stream.wait_event(req.uploaded)  # chunks on other streams read the upload
         a = req.next_row
@@ -249,17 +288,33 @@ class MappingServer:
             self._free_streams.append(sid)
             if kind == "chunk":
                 req.outstanding -= 1
+                if req.fut.cancelled() and req.outstanding == 0:
+                    self._release(req)
             else:
                 r = req.res
                 r.indices = req.h_idx.numpy().astype(np.uint64)
                 r.distances = req.h_dist.numpy()
                 r.scores = req.h_scores.numpy()[0]
                 r.done = time.perf_counter()
-                self._used -= req.need
-                self._active.remove(req)
-                req.dev = req.idx = req.dist = None
-                req.fut.set_result(r)
+                self._release(req)
+                try:
+                    req.fut.set_result(r)
+                except cf.InvalidStateError:  # cancelled while its last step ran
+                    pass
         self._inflight = keep
+
+    def _release(self, req: _Req):
+        """Frees an admitted request's memory and slot (its work has completed)."""
+        if req in self._active:
+            self._used -= req.need
+            self._active.remove(req)
+        req.dev = req.idx = req.dist = None
+
+    def _drop_cancelled(self):
+        """Admitted requests cancelled with nothing in flight are released at once."""
+        for req in [r for r in self._active if r.fut.cancelled() and r.outstanding == 0
+                    and not getattr(r, "final_issued", False)]:
+            self._release(req)
 
     def _engine(self):
         torch.cuda.set_device(0)
@@ -273,9 +328,12 @@ class MappingServer:
                     continue
             progressed = False
             self._retire()
+            with self._lock:
+                self._drop_cancelled()
             while self._free_streams:
                 # finished requests first: their score + copy-back frees a slot
                 fin = next((r for r in self._active if r.next_row >= r.n and r.outstanding == 0
+                            and not r.fut.cancelled()
                             and not getattr(r, "final_issued", False)), None)
                 if fin is not None:
                     fin.final_issued = True
@@ -288,4 +346,6 @@ class MappingServer:
                 self._issue_chunk(req, self._free_streams.popleft())
                 progressed = True
             if not progressed:
-                time.sleep(0.0002)
+                with self._lock:
+                    if not self._stop:
+                        self._lock.wait(timeout=self.poll_s)

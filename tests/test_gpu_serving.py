@@ -111,3 +111,37 @@ def test_empty_and_single_cell(server_setup):
     with pytest.raises(ValueError):
         _server(ann, src_ids, q_ids).submit(CSR(np.zeros(0, np.uint32), np.zeros(0, np.int32),
                                                 np.zeros(1, np.int64), 3))
+
+
+def test_cancel_frees_slot_and_memory(server_setup):
+    """A cancelled queued request never runs; a cancelled admitted request stops issuing
+    chunks and gives back its memory and slot, and the requests around it still get the
+    same answers as alone."""
+    ann, src_ids, q_ids, pool = server_setup
+    small = _req(pool, 100, 160)
+    big = _req(pool, 0, 8000)
+    with _server(ann, src_ids, q_ids) as srv:
+        alone = srv.submit(small).result(timeout=30)
+    probe = _server(ann, src_ids, q_ids)
+    # room for the big request only: the next one queues behind it
+    budget = probe.footprint(big) + probe.footprint(small) // 2
+    with _server(ann, src_ids, q_ids, mem_budget=budget, chunk_rows=16) as srv:
+        fb = srv.submit(big)
+        fq = srv.submit(small)  # queued: does not fit beside the big one
+        assert srv.cancel(fq)
+        # cancel the big one once its first chunks are on the GPU (500 chunks in all)
+        deadline = time.time() + 30
+        while not any(r.next_row > 0 for r in list(srv._active)) and time.time() < deadline:
+            time.sleep(0.0005)
+        assert srv.cancel(fb)
+        after = srv.submit(small).result(timeout=60)
+        assert fb.cancelled() and fq.cancelled()
+        # everything drained: no memory or slot left behind by the cancelled requests
+        deadline = time.time() + 30
+        while (srv._active or srv._inflight) and time.time() < deadline:
+            time.sleep(0.01)
+        assert srv._used == 0 and not srv._active
+    assert after.error is None
+    np.testing.assert_array_equal(after.indices, alone.indices)
+    np.testing.assert_array_equal(after.distances, alone.distances)
+    np.testing.assert_array_equal(after.scores, alone.scores)
