@@ -243,8 +243,9 @@ def chebyshev_subspace(dev, n_top: int, tol: float, dyn: float = 16.0,
                        max_iter: int = 100):
     """Host loop of Chebyshev-filtered block subspace iteration over a backend ``dev``
     providing ``rayleigh_ritz() -> (theta, res)`` (active block, descending),
-    ``lock(c, theta)``, ``filter(m, center, half_width, scale)``, ``orthonormalize()``
-    and ``result(n_active) -> (w, Vt, res, orth)``.
+    ``lock(c, theta)``, ``filter(m, center, half_width, scale)``,
+    ``orth_rayleigh_ritz() -> (theta, res)`` (one CholeskyQR pass, then the Ritz pairs of
+    the block from H q = theta G q) and ``result(n_active) -> (w, Vt, res, orth)``.
 
     Each iteration filters the active block with a Chebyshev polynomial that damps
     [0, cut] (the merge matrix is positive semi-definite; cut is the block's smallest
@@ -300,7 +301,10 @@ def chebyshev_subspace(dev, n_top: int, tol: float, dyn: float = 16.0,
         m = int(dyn / math.acosh(max(x_up, 1.0 + 1e-12)))
         m = max(2, min(max_degree if res[0] <= 1e-3 * th[0] else 4, m))
         dev.filter(m, half, half, lam_up)
-        dev.orthonormalize()
+        # one CholeskyQR pass, then the Rayleigh-Ritz step as H q = theta G q with the
+        # block's Gram G (close to I after that pass): the second CholeskyQR pass folded
+        # into the small eigenproblem, one host round trip fewer per iteration
+        pending = dev.orth_rayleigh_ritz()
     raise EigenNotConverged(f"no convergence in {max_iter} filtered iterations")
 
 
@@ -395,34 +399,50 @@ class _DeviceBlock:
                 return i
         raise RuntimeError("no spare block buffer")
 
-    def orthonormalize(self):
+    def _cholqr_pass(self):
         n = self.nb
-        for _ in range(2):
-            X = self.buf[self.X]
-            G = self._gram(X, X, n).cpu().numpy()
-            G = 0.5 * (G + G.T)
-            try:
-                R = _sla.cholesky(G, lower=False)  # G = R^T R (scipy: see rayleigh_ritz)
-            except np.linalg.LinAlgError as exc:
-                raise EigenNotConverged("filtered block lost rank (CholeskyQR)") from exc
-            Rinv, info = _sla.lapack.dtrtri(R, lower=0)
-            if info != 0:
-                raise EigenNotConverged("filtered block lost rank (CholeskyQR)")
-            o = self._spare(self.X)
-            self._times_small(X, Rinv, self.buf[o])
-            self.X = o
+        X = self.buf[self.X]
+        G = self._gram(X, X, n).cpu().numpy()
+        G = 0.5 * (G + G.T)
+        try:
+            R = _sla.cholesky(G, lower=False)  # G = R^T R (scipy: see rayleigh_ritz)
+        except np.linalg.LinAlgError as exc:
+            raise EigenNotConverged("filtered block lost rank (CholeskyQR)") from exc
+        Rinv, info = _sla.lapack.dtrtri(R, lower=0)
+        if info != 0:
+            raise EigenNotConverged("filtered block lost rank (CholeskyQR)")
+        o = self._spare(self.X)
+        self._times_small(X, Rinv, self.buf[o])
+        self.X = o
 
-    def rayleigh_ritz(self):
+    def orthonormalize(self):
+        self._cholqr_pass()
+        self._cholqr_pass()
+
+    def orth_rayleigh_ritz(self):
+        self._cholqr_pass()
+        return self.rayleigh_ritz(generalized=True)
+
+    def rayleigh_ritz(self, generalized: bool = False):
         n, cols = self.nb, self._cols()
         X = self.buf[self.X]
         xm = self._spare(self.X)
         XM = self.buf[xm]
         _lib.call("sh_symm_block", ptr(self.Mw), self.d, ptr(X), None, cols, self.ld, 1.0, 0.0,
                   0.0, ptr(XM), ptr(self.ws_sb), self.nb_sb, self.st)
-        H = self._gram(X, XM, n).cpu().numpy()
+        Hd = self._gram(X, XM, n).clone()
+        if generalized:  # the block's own Gram, read back with H in one copy
+            Gd = self._gram(X, X, n)
+            HG = torch.cat([Hd, Gd]).cpu().numpy()
+            H, G = HG[:n], HG[n:]
+        else:
+            H, G = Hd.cpu().numpy(), None
         # scipy's LAPACK binding: numpy's eigh and cholesky are an order of magnitude
         # slower at these sizes on a many-threaded host
-        w, Q = _sla.eigh(0.5 * (H + H.T))
+        try:
+            w, Q = _sla.eigh(0.5 * (H + H.T), None if G is None else 0.5 * (G + G.T))
+        except np.linalg.LinAlgError as exc:
+            raise EigenNotConverged("filtered block lost rank (Rayleigh-Ritz)") from exc
         w, Q = w[::-1].copy(), np.ascontiguousarray(Q[:, ::-1])
         xo = self._spare(self.X, xm)
         mo = self._spare(self.X, xm, xo)

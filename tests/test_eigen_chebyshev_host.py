@@ -24,20 +24,28 @@ class _NumpyBlock:
         self.filter_degrees = []
         self.orthonormalize()
 
-    def orthonormalize(self):
+    def _cholqr_pass(self):
         n = self.nb
-        for _ in range(2):
-            X = self.X[:, :n]
-            G = X.T @ X
-            R = np.linalg.cholesky(0.5 * (G + G.T)).T
-            self.X[:, :n] = X @ sla.solve_triangular(R, np.eye(n), lower=False)
+        X = self.X[:, :n]
+        G = X.T @ X
+        R = np.linalg.cholesky(0.5 * (G + G.T)).T
+        self.X[:, :n] = X @ sla.solve_triangular(R, np.eye(n), lower=False)
 
-    def rayleigh_ritz(self):
+    def orthonormalize(self):
+        self._cholqr_pass()
+        self._cholqr_pass()
+
+    def orth_rayleigh_ritz(self):
+        self._cholqr_pass()
+        return self.rayleigh_ritz(generalized=True)
+
+    def rayleigh_ritz(self, generalized=False):
         n = self.nb
         X = self.X[:, :n]
         XM = self.Mw @ X
         H = X.T @ XM
-        w, Q = np.linalg.eigh(0.5 * (H + H.T))
+        G = X.T @ X if generalized else None
+        w, Q = sla.eigh(0.5 * (H + H.T), None if G is None else 0.5 * (G + G.T))
         w, Q = w[::-1].copy(), Q[:, ::-1]
         self.X[:, :n] = X @ Q
         XM = XM @ Q
