@@ -210,7 +210,7 @@ def generate_csr_gpu(
         m = s[:, None] * mu[None, :] * fold[grp]
         shape = (1.0 / phi).expand(nb, d)
         # Gamma(shape=1/phi, scale=phi*m) then Poisson: NB(1/phi, 1/(1+phi m))
-        lam = torch._standard_gamma(shape.contiguous()) * (phi[None, :] * m)
+        lam = torch._standard_gamma(shape.contiguous(), generator=g) * (phi[None, :] * m)
         del m
         c = torch.poisson(lam, generator=g)
         del lam
