@@ -109,6 +109,29 @@ int sh_gram(const void *Z, int z_f64, int64_t n, int32_t d, int64_t ld, int ones
             size_t workspace_bytes, sh_stream_t stream);
 size_t sh_gram_workspace(int64_t n, int32_t d, int z_f64);
 
+/* The same G = Z^T Z and column sums as sh_densify + sh_gram (z = clip((x - center) *
+ * scale) over rows[0..n_rows), NULL: rows 0..n_rows-1), computed from the CSR entries
+ * without densifying z: with c_f the z of a zero count and delta = z - c non-zero only
+ * at stored entries, Z^T Z = delta^T delta + c s^T + s c^T + n c c^T (s = delta^T 1).
+ * delta^T delta is summed exactly in int64 fixed point (per-feature power-of-two scales
+ * from the column maxima of |delta|, products rounded once to 2^-F of those scales with
+ * F <= 50 set by the row-slab size), so the result is bit-reproducible; the slab sums and
+ * the rank-one terms are added in FP64 in a fixed order.  `nnz` bounds the stored
+ * entries of the selected rows (it sizes the packed entries; the call checks it against
+ * the rows' true count, which costs one stream synchronisation).  d + 1 <= 8064; col_map
+ * must be injective, as for sh_densify.  A non-finite delta turns its feature's row and
+ * column of G into NaN.  accumulate != 0 adds to G_out / colsum_out.
+ * Replaces: the Gram of the augmented chunk whose SVD IncrementalPCA.partial_fit takes
+ *           (_incremental_pca.py:345-363, called from ann.py:241,250), on z of
+ *           ann.py:191-192. */
+int sh_sparse_gram(const uint32_t *vals, const int32_t *cols, const int64_t *rowptr,
+                   const int64_t *rows, int64_t n_rows, int64_t nnz, const int32_t *col_map,
+                   int32_t n_cols_in, int32_t d, double sf, int log_transform,
+                   const double *libsize, const double *center, const double *scale,
+                   double clip, int accumulate, double *G_out, double *colsum_out,
+                   void *workspace, size_t workspace_bytes, sh_stream_t stream);
+size_t sh_sparse_gram_workspace(int64_t n_rows, int64_t nnz, int32_t d);
+
 /* ---- symmetric eigensolve of the PCA merge matrix (FP64) --------------------------
  * The top (dims + 1) eigenpairs of the d x d merge matrix replace the SVD of the
  * augmented chunk in IncrementalPCA.partial_fit (_incremental_pca.py:345-363, called

@@ -44,6 +44,9 @@ _SIGS = {
     "sh_densify": (_int, [_p, _p, _p, _p, _i64, _p, _i32, _i32, _f64, _int, _p, _p, _p, _f64, _int, _p, _i64, _p]),
     "sh_gram": (_int, [_p, _int, _i64, _i32, _i64, _int, _int, _p, _p, _p, _sz, _p]),
     "sh_gram_workspace": (_sz, [_i64, _i32, _int]),
+    "sh_sparse_gram": (_int, [_p, _p, _p, _p, _i64, _i64, _p, _i32, _i32, _f64, _int, _p, _p,
+                              _p, _f64, _int, _p, _p, _p, _sz, _p]),
+    "sh_sparse_gram_workspace": (_sz, [_i64, _i64, _i32]),
     "sh_merge_matrix": (_int, [_p, _i32, _f64, _p, _p, _p, _i32, _p, _p, _p]),
     "sh_lanczos_workspace": (_sz, [_i32, _i32]),
     "sh_lanczos_steps": (_int, [_p, _i32, _p, _i32, _i32, _i32, _p, _p, _p, _p, _sz, _p]),

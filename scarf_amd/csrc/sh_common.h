@@ -82,6 +82,31 @@ __device__ __forceinline__ float wave_max_f32(float v) {
   return v;
 }
 
+// Normalised count x = log1p(sf * c / l) in FP64 with an IEEE division, as numpy evaluates
+// it (assay.py:54-64).
+__device__ __forceinline__ double norm_value(uint32_t c, double sf, double l, int log_t) {
+  double v = (sf * (double)c) / l;
+  return log_t ? log1p(v) : v;
+}
+
+// Feature of input column c (identity without a map); -1 when c is out of range or unmapped.
+__device__ __forceinline__ int map_col(const int32_t *col_map, int32_t c, int32_t n_in) {
+  if (c < 0 || c >= n_in) return -1;
+  return col_map ? col_map[c] : c;
+}
+
+// Tile t of the row-major enumeration of an nb x nb upper block triangle (row bi holds
+// nb - bi tiles) -> block coordinates (bi, bj), bi <= bj.
+__device__ __forceinline__ void tile_coords(int t, int nb, int &bi, int &bj) {
+  int r = 0, base = 0;
+  while (t >= base + (nb - r)) {
+    base += nb - r;
+    ++r;
+  }
+  bi = r;
+  bj = r + (t - base);
+}
+
 inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 

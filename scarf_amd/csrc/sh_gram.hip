@@ -26,17 +26,6 @@ typedef float floatx16 __attribute__((ext_vector_type(16)));
 constexpr int kGT = 128;  // output tile edge
 constexpr int kGK = 16;   // rows per LDS stage
 
-__device__ __forceinline__ void tile_coords(int t, int nb, int &bi, int &bj) {
-  // row-major enumeration of the upper triangle: row bi has nb - bi tiles
-  int r = 0, base = 0;
-  while (t >= base + (nb - r)) {
-    base += nb - r;
-    ++r;
-  }
-  bi = r;
-  bj = r + (t - base);
-}
-
 __global__ void __launch_bounds__(256) k_gram(const float *__restrict__ Z, int64_t n, int64_t ld,
                                               int nb, int T, int S, int64_t rows_per_slab,
                                               float *__restrict__ part) {

@@ -11,17 +11,6 @@
 
 namespace sh {
 
-// x = log1p(sf * c / l) in FP64 with an IEEE division, as numpy evaluates it.
-__device__ __forceinline__ double norm_value(uint32_t c, double sf, double l, int log_t) {
-  double v = (sf * (double)c) / l;
-  return log_t ? log1p(v) : v;
-}
-
-__device__ __forceinline__ int map_col(const int32_t *col_map, int32_t c, int32_t n_in) {
-  if (c < 0 || c >= n_in) return -1;
-  return col_map ? col_map[c] : c;
-}
-
 // One wave per row: library size over the mapped columns; 0 -> 1.
 __global__ void __launch_bounds__(256) k_libsize(const uint32_t *__restrict__ vals,
                                                  const int32_t *__restrict__ cols,

@@ -44,6 +44,11 @@ except ImportError:  # pragma: no cover
 # Krylov basis would span most of the space
 EIG_DEVICE_MIN_D = 512
 
+# the sparse Gram forms sum_r nnz_r^2 / 2 products with LDS atomics where the dense one
+# forms n d^2 / 2 on FP64 MFMA; it is taken up to this density (nnz / (n d))
+SPARSE_GRAM_MAX_DENSITY = 0.3
+SPARSE_GRAM_MAX_D = 8063
+
 
 def _top_eigh(M: np.ndarray, n_top: int):
     """Top ``n_top`` eigenpairs of a symmetric FP64 matrix, descending."""
@@ -228,12 +233,16 @@ def _z_params(stats: FeatureStats, disable_scaling: bool):
 def gram_of_rows(csr: DeviceCSR, stats: FeatureStats, params: NormParams, rows,
                  col_map=None, disable_scaling=False, sub_rows: int = 32768,
                  precision: str = "fp64", stream=None, to_host: bool = True,
-                 center=None, scale=None):
+                 center=None, scale=None, method: str = "auto"):
     """(Z^T Z, column sums, n) of z over ``rows`` (device int64 tensor) in FP64: host
     arrays, or with ``to_host=False`` the Gram as a device tensor (column sums on host).
 
-    ``precision="fp64"`` densifies z in FP64 and uses FP64 MFMA (parity with the
-    reference's FP64 PCA on every component); ``"fp32"`` rounds z to FP32 first.
+    ``method="sparse"`` sums the products of the stored entries only (``sh_sparse_gram``:
+    z = c + delta with c the z of a zero count, delta^T delta in exact int64 fixed point);
+    ``"dense"`` densifies z (``sh_densify``) and multiplies on MFMA (``sh_gram``): FP64 z
+    and FP64 MFMA with ``precision="fp64"`` (parity with the reference's FP64 PCA on every
+    component), z rounded to FP32 with ``"fp32"``.  ``"auto"`` takes the sparse form for
+    FP64 when the rows are at most SPARSE_GRAM_MAX_DENSITY dense.
     z = (x - center) * scale, by default the stats' mu and 1 / sigma; ``center`` /
     ``scale`` (device FP64 [d]) replace them.
     """
@@ -245,16 +254,33 @@ def gram_of_rows(csr: DeviceCSR, stats: FeatureStats, params: NormParams, rows,
     f64 = precision == "fp64"
     if precision not in ("fp64", "fp32"):
         raise ValueError(f"unknown Gram precision {precision!r}")
+    if method not in ("auto", "sparse", "dense"):
+        raise ValueError(f"unknown Gram method {method!r}")
+    if method == "sparse" and not f64:
+        raise ValueError("the sparse Gram is FP64 only")
     G = torch.zeros((d, d), dtype=torch.float64, device=dev)
     cs = torch.zeros(d, dtype=torch.float64, device=dev)
     c0, s0 = _z_params(stats, disable_scaling)
     center = c0 if center is None else center
     scale = s0 if scale is None else scale
     st = stream_handle(stream)
+    clip = float(params.clip) if params.clip else 0.0
+    if method != "dense" and f64 and d <= SPARSE_GRAM_MAX_D:
+        nnz = int((csr.rowptr[rows + 1] - csr.rowptr[rows]).sum().item()) if n else 0
+        if method == "sparse" or nnz <= SPARSE_GRAM_MAX_DENSITY * n * d:
+            ws = workspace("gram")
+            buf, nbytes = ws.get(_lib.lib().sh_sparse_gram_workspace(n, nnz, d))
+            _lib.call("sh_sparse_gram", ptr(csr.vals), ptr(csr.cols), ptr(csr.rowptr),
+                      ptr(rows), n, nnz, ptr(col_map), csr.n_cols, d,
+                      float(params.size_factor), int(params.log_transform),
+                      ptr(stats.libsize), ptr(center), ptr(scale), clip, 0, ptr(G), ptr(cs),
+                      ptr(buf), nbytes, st)
+            return (G.cpu().numpy() if to_host else G), cs.cpu().numpy(), n
+    elif method == "sparse":
+        raise ValueError(f"the sparse Gram takes at most {SPARSE_GRAM_MAX_D} features, not {d}")
     zb = min(n, sub_rows)
     Z = torch.empty((max(zb, 1), ld), dtype=torch.float64 if f64 else torch.float32, device=dev)
     ws = workspace("gram")
-    clip = float(params.clip) if params.clip else 0.0
     for a in range(0, n, sub_rows):
         b = min(n, a + sub_rows)
         _lib.call("sh_densify", ptr(csr.vals), ptr(csr.cols), ptr(csr.rowptr), ptr(rows, a),

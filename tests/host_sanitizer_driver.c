@@ -44,6 +44,7 @@ int main(void) {
       sink += sh_norm_stats_workspace(n, -1, d);
       sink += sh_gram_workspace(n, d, 1);
       sink += sh_gram_workspace(n, d, 0);
+      sink += sh_sparse_gram_workspace(n, n > 0 ? n * 21 : 0, d);
       sink += sh_lanczos_workspace(d, 192);
       sink += sh_block_gram_workspace(d, 94, 94);
       sink += sh_symm_block_workspace(d, 96, 96);
@@ -59,7 +60,7 @@ int main(void) {
       sink += sh_graph_csr_workspace(n, n * 21, 21, 11);
       sink += sh_spd_sqrt_workspace(d);
       sink += (size_t)sh_knn_pack_k(SH_BF16, d) + (size_t)sh_knn_pack_row_bytes(SH_FP8, d);
-      n_checked += 20;
+      n_checked += 21;
     }
   }
   if (sh_knn_pack_k(99, 50) >= 0 || sh_knn_pack_row_bytes(99, 50) >= 0) {
@@ -80,6 +81,14 @@ int main(void) {
                  NULL, 127, st));
   BAD(sh_gram(NULL, 1, 100, 50, 100, 1, 0, NULL, NULL, ws, wsb, st));
   BAD(sh_gram(NULL, 1, -1, 50, 128, 1, 0, NULL, NULL, ws, wsb, st));
+  BAD(sh_sparse_gram(NULL, NULL, NULL, NULL, 10, 100, NULL, 50, 50, 1e4, 1, NULL, NULL, NULL,
+                     0.0, 0, NULL, NULL, ws, wsb, st));
+  BAD(sh_sparse_gram((const uint32_t *)8, (const int32_t *)8, (const int64_t *)8, NULL, 10, 100,
+                     NULL, 9000, 9000, 1e4, 1, (const double *)8, (const double *)8,
+                     (const double *)8, 0.0, 0, (double *)8, (double *)8, ws, wsb, st));
+  BAD(sh_sparse_gram((const uint32_t *)8, (const int32_t *)8, (const int64_t *)8, NULL, 10, 100,
+                     NULL, 50, 50, 1e4, 1, (const double *)8, (const double *)8,
+                     (const double *)8, 0.0, 0, (double *)8, (double *)8, ws, wsb, st));
   BAD(sh_merge_matrix(NULL, 50, 10.0, NULL, NULL, NULL, 0, NULL, NULL, st));
   BAD(sh_lanczos_steps(NULL, 50, NULL, 64, 0, 2, NULL, NULL, NULL, ws, wsb, st));
   BAD(sh_dgemm(0, -1, 4, 4, 1.0, NULL, 4, NULL, 4, 0.0, NULL, 4, st));

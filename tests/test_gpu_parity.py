@@ -65,8 +65,9 @@ def test_norm_stats_deterministic(cuda, data):
     assert bool((a.mu == b.mu).all()) and bool((a.sigma == b.sigma).all())
 
 
-@pytest.mark.parametrize("precision", ["fp64", "fp32"])
-def test_gram_parity(cuda, data, precision):
+@pytest.mark.parametrize("precision,method", [("fp64", "dense"), ("fp32", "dense"),
+                                              ("fp64", "sparse")])
+def test_gram_parity(cuda, data, precision, method):
     import torch
     from scarf_amd.device import DeviceCSR
     from scarf_amd.normalize import NormParams, feature_stats
@@ -76,8 +77,8 @@ def test_gram_parity(cuda, data, precision):
     p = NormParams()
     st = feature_stats(dev, 1000, p)
     rows = torch.arange(100, 2900, device="cuda", dtype=torch.int64)
-    say("gram rows 100..2900 d", csr.n_cols, precision)
-    G, cs, n = gram_of_rows(dev, st, p, rows, sub_rows=1000, precision=precision)
+    say("gram rows 100..2900 d", csr.n_cols, precision, method)
+    G, cs, n = gram_of_rows(dev, st, p, rows, sub_rows=1000, precision=precision, method=method)
     x = onorm.normed(dense.astype(np.float64), sf=1000.0)
     mu, sigma = onorm.mu_sigma(x)
     z = ((x - mu) / sigma)[100:2900]
