@@ -191,92 +191,126 @@ __device__ __forceinline__ int sg_bcast32(int v) {
   return __builtin_amdgcn_update_dpp(0, v, 0x150 + B, 0xF, 0xF, true);
 }
 
-// A wave's batch of 64 rows, laid out so that group G (0..15) is rows 4G..4G+3: lane L holds
-// row 4 (L % 16) + L / 16, so the four rows of group G sit in lane G of each 16-lane slot
-// and one row_newbcast:G hands each slot its row's segment bounds.  Rows with more than 16
-// entries in either block, or none in one, carry empty segments here.
+// A row's two segments in the tile: the "lane" side L (one entry per lane) and the
+// "broadcast" side B (one entry per step, broadcast to the lanes).  The cell of a product
+// is (J column) * 129 + (K column); mulL / mulB carry each side's byte stride so that either
+// segment can take either role.
+constexpr int kSgMulJ = kSgLd * 8, kSgMulK = 8;
+
+// A wave's batch of 64 rows, laid out so that group G (0..15) is rows 4G..4G+3: lane L
+// holds row 4 (L % 16) + L / 16, so the four rows of group G sit in lane G of each 16-lane
+// slot.  After each group the batch rotates by one lane within the slots, so the next
+// group's rows are always in lane 0 and one row_newbcast:0 hands each slot its row's
+// segments.  Rows outside the group path (both segments longer than 16, or one empty)
+// carry nL = nB = 0 here.
 struct SgBatch {
-  int64_t pj, pk;
-  int nj, nk;
-  int mk;  // largest nk over the lane's group (lanes L, L^16, L^32, L^48)
+  int64_t pL, pB;
+  int nL, nB, mulL, mulB;
+  int mB;  // largest nB over the lane's group (lanes L, L^16, L^32, L^48)
 };
 
-// One group: slot s = lane / 16 holds its row's J entry a = lane % 16 and K entry a.  The
-// values are the raw loads (consumed a group later); masking happens in sg_group_products.
+// One group: slot s = lane / 16 holds its row's lane-side entry a = lane % 16 and
+// broadcast-side entry a of the first 16-entry chunk.  Values are the raw loads (consumed
+// a group later); masking happens in sg_group_products.
 struct SgGroup {
-  double vJ, vK;
-  int fJ, fK;
-  int nj, nk;  // the slot row's counts
-  int mk;      // largest nk of the group (uniform)
+  double vL, vB;
+  int fL, fB;
+  int64_t pB;
+  int nL, nB, mulL, mulB;  // the slot row's
+  int mB;                  // largest nB of the group (uniform)
 };
 
-template <int G>
-__device__ __forceinline__ void sg_fetch(const SgBatch &r, SgGroup &g,
-                                         const double *__restrict__ dv,
-                                         const uint16_t *__restrict__ fc, int a) {
-  const int64_t pj = __double_as_longlong(sg_bcast64<G>(__longlong_as_double(r.pj)));
-  const int64_t pk = __double_as_longlong(sg_bcast64<G>(__longlong_as_double(r.pk)));
-  g.nj = sg_bcast32<G>(r.nj);
-  g.nk = sg_bcast32<G>(r.nk);
-  g.mk = __builtin_amdgcn_readlane(r.mk, G);
-  // unconditional loads (entry 0 always exists) keep the wait counts exact
-  const int64_t iJ = a < g.nj ? pj + a : 0, iK = a < g.nk ? pk + a : 0;
-  g.vJ = dv[iJ];
-  g.fJ = fc[iJ];
-  g.vK = dv[iK];
-  g.fK = fc[iK];
+// lane (i + 1) % 16 of each 16-lane row -> lane i (DPP row_ror:15)
+__device__ __forceinline__ int sg_rot32(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, 0x12F, 0xF, 0xF, false);
+}
+__device__ __forceinline__ int64_t sg_rot64(int64_t v) {
+  const uint32_t lo = (uint32_t)sg_rot32((int)(uint32_t)v);
+  const uint32_t hi = (uint32_t)sg_rot32((int)(uint32_t)((uint64_t)v >> 32));
+  return (int64_t)(((uint64_t)hi << 32) | lo);
 }
 
-// Adds the products of K entry B with the slot's J entries: the K value and column are
-// broadcast from lane B of the slot.  Slots whose row has fewer K entries broadcast a
-// zero value (a zero product adds nothing).
+// Takes the group in lane 0 of each slot, issues its loads, rotates the batch.
+__device__ __forceinline__ void sg_fetch(SgBatch &r, SgGroup &g, const double *__restrict__ dv,
+                                         const uint16_t *__restrict__ fc, int a) {
+  const int64_t pL = __double_as_longlong(sg_bcast64<0>(__longlong_as_double(r.pL)));
+  g.pB = __double_as_longlong(sg_bcast64<0>(__longlong_as_double(r.pB)));
+  g.nL = sg_bcast32<0>(r.nL);
+  g.nB = sg_bcast32<0>(r.nB);
+  g.mulL = sg_bcast32<0>(r.mulL);
+  g.mulB = sg_bcast32<0>(r.mulB);
+  g.mB = __builtin_amdgcn_readfirstlane(r.mB);
+  // unconditional loads (entry 0 always exists) keep the wait counts exact
+  const int64_t iL = a < g.nL ? pL + a : 0, iB = a < g.nB ? g.pB + a : 0;
+  g.vL = dv[iL];
+  g.fL = fc[iL];
+  g.vB = dv[iB];
+  g.fB = fc[iB];
+  r.pL = sg_rot64(r.pL);
+  r.pB = sg_rot64(r.pB);
+  r.nL = sg_rot32(r.nL);
+  r.nB = sg_rot32(r.nB);
+  r.mulL = sg_rot32(r.mulL);
+  r.mulB = sg_rot32(r.mulB);
+  r.mB = sg_rot32(r.mB);
+}
+
+// Adds the products of broadcast entry B of the chunk with the lanes' entries: the value
+// and byte offset are broadcast from lane B of each 16-lane row.  Lanes whose row has no
+// entry B (B >= nb) or no lane entry (act false) add nothing.
 template <int B>
-__device__ __forceinline__ void sg_kstep(unsigned long long *acc, double vJ, double vK, int oJ,
-                                         int oK, bool act) {
-  const double kb = sg_bcast64<B>(vK);
-  const int o = oJ + sg_bcast32<B>(oK);  // byte offset into the tile
-  const double t = fma(vJ, kb, kSgMagic);
+__device__ __forceinline__ void sg_kstep(unsigned long long *acc, double vL, double vB, int oL,
+                                         int oB, bool act, int nb) {
+  const double kb = sg_bcast64<B>(vB);
+  const int o = oL + sg_bcast32<B>(oB);
+  const double t = fma(vL, kb, kSgMagic);
   const long long q = __double_as_longlong(t) - __double_as_longlong(kSgMagic);
-  if (act)
+  if (act && B < nb)
     atomicAdd(reinterpret_cast<unsigned long long *>(reinterpret_cast<char *>(acc) + o),
               (unsigned long long)q);
 }
 
-__device__ __forceinline__ void sg_group_products(unsigned long long *acc, SgGroup &g, int a) {
-  if (g.mk <= 0) return;
-  // the loads must land here, not where they were issued: keeps the arithmetic below
-  // from being hoisted next to them
-  asm volatile("" : "+v"(g.vJ), "+v"(g.vK), "+v"(g.fJ), "+v"(g.fK));
-  const double vJ = g.vJ, vK = a < g.nk ? g.vK : 0.0;
-  const int oJ = (g.fJ & (kSgB - 1)) * (kSgLd * 8), oK = (g.fK & (kSgB - 1)) * 8;
-  const bool act = a < g.nj;
-  sg_kstep<0>(acc, vJ, vK, oJ, oK, act);
-  if (g.mk > 1) sg_kstep<1>(acc, vJ, vK, oJ, oK, act);
-  if (g.mk > 2) sg_kstep<2>(acc, vJ, vK, oJ, oK, act);
-  if (g.mk > 3) sg_kstep<3>(acc, vJ, vK, oJ, oK, act);
-  if (g.mk > 4) sg_kstep<4>(acc, vJ, vK, oJ, oK, act);
-  if (g.mk > 5) sg_kstep<5>(acc, vJ, vK, oJ, oK, act);
-  if (g.mk > 6) sg_kstep<6>(acc, vJ, vK, oJ, oK, act);
-  if (g.mk > 7) sg_kstep<7>(acc, vJ, vK, oJ, oK, act);
-  if (g.mk > 8) sg_kstep<8>(acc, vJ, vK, oJ, oK, act);
-  if (g.mk > 9) sg_kstep<9>(acc, vJ, vK, oJ, oK, act);
-  if (g.mk > 10) sg_kstep<10>(acc, vJ, vK, oJ, oK, act);
-  if (g.mk > 11) sg_kstep<11>(acc, vJ, vK, oJ, oK, act);
-  if (g.mk > 12) sg_kstep<12>(acc, vJ, vK, oJ, oK, act);
-  if (g.mk > 13) sg_kstep<13>(acc, vJ, vK, oJ, oK, act);
-  if (g.mk > 14) sg_kstep<14>(acc, vJ, vK, oJ, oK, act);
-  if (g.mk > 15) sg_kstep<15>(acc, vJ, vK, oJ, oK, act);
+// Steps 0..n-1 (n uniform, <= 16) of one broadcast chunk.
+__device__ __forceinline__ void sg_chunk(unsigned long long *acc, double vL, double vB, int oL,
+                                         int oB, bool act, int nb, int n) {
+  sg_kstep<0>(acc, vL, vB, oL, oB, act, nb);
+  if (n > 1) sg_kstep<1>(acc, vL, vB, oL, oB, act, nb);
+  if (n > 2) sg_kstep<2>(acc, vL, vB, oL, oB, act, nb);
+  if (n > 3) sg_kstep<3>(acc, vL, vB, oL, oB, act, nb);
+  if (n > 4) sg_kstep<4>(acc, vL, vB, oL, oB, act, nb);
+  if (n > 5) sg_kstep<5>(acc, vL, vB, oL, oB, act, nb);
+  if (n > 6) sg_kstep<6>(acc, vL, vB, oL, oB, act, nb);
+  if (n > 7) sg_kstep<7>(acc, vL, vB, oL, oB, act, nb);
+  if (n > 8) sg_kstep<8>(acc, vL, vB, oL, oB, act, nb);
+  if (n > 9) sg_kstep<9>(acc, vL, vB, oL, oB, act, nb);
+  if (n > 10) sg_kstep<10>(acc, vL, vB, oL, oB, act, nb);
+  if (n > 11) sg_kstep<11>(acc, vL, vB, oL, oB, act, nb);
+  if (n > 12) sg_kstep<12>(acc, vL, vB, oL, oB, act, nb);
+  if (n > 13) sg_kstep<13>(acc, vL, vB, oL, oB, act, nb);
+  if (n > 14) sg_kstep<14>(acc, vL, vB, oL, oB, act, nb);
+  if (n > 15) sg_kstep<15>(acc, vL, vB, oL, oB, act, nb);
 }
 
-// Groups G..15 with one group of loads in flight: fetch group G into `nxt`, multiply `cur`.
-template <int G>
-__device__ __forceinline__ void sg_pipe(unsigned long long *acc, const SgBatch &r,
-                                        SgGroup &cur, SgGroup &nxt,
-                                        const double *__restrict__ dv,
-                                        const uint16_t *__restrict__ fc, int a) {
-  if constexpr (G < 16) sg_fetch<G>(r, nxt, dv, fc, a);
-  sg_group_products(acc, cur, a);
-  if constexpr (G < 16) sg_pipe<G + 1>(acc, r, nxt, cur, dv, fc, a);
+__device__ __forceinline__ void sg_group_products(unsigned long long *acc, SgGroup &g,
+                                                  const double *__restrict__ dv,
+                                                  const uint16_t *__restrict__ fc, int a) {
+  if (g.mB <= 0) return;
+  // the loads must land here, not where they were issued: keeps the arithmetic below
+  // from being hoisted next to them
+  asm volatile("" : "+v"(g.vL), "+v"(g.vB), "+v"(g.fL), "+v"(g.fB));
+  const bool act = a < g.nL;
+  const int oL = (g.fL & (kSgB - 1)) * g.mulL;
+  sg_chunk(acc, g.vL, a < g.nB ? g.vB : 0.0, oL, (g.fB & (kSgB - 1)) * g.mulB, act, g.nB,
+           min(g.mB, 16));
+  // broadcast sides longer than 16: further chunks, loaded here
+  for (int c = 16; c < g.mB; c += 16) {
+    const int nb = g.nB - c;
+    const int64_t iB = a < nb ? g.pB + c + a : 0;
+    const double vB = dv[iB];
+    const int fB = fc[iB];
+    sg_chunk(acc, g.vL, a < nb ? vB : 0.0, oL, (fB & (kSgB - 1)) * g.mulB, act, nb,
+             min(g.mB - c, 16));
+  }
 }
 
 __global__ void __launch_bounds__(kSgThreads) k_sg_tile(
@@ -305,29 +339,6 @@ __global__ void __launch_bounds__(kSgThreads) k_sg_tile(
   const int64_t r1 = min(n, r0 + rows_per_slab);
   const int a = lane & 15;
 
-  // nJ x nK products of one row's segments flattened over the lanes (nj, nk <= 64; lanes
-  // past nj / nk hold zeros)
-  auto products = [&](double aJ, int fJ, double aK, int fK, int nj, int nk) {
-    const int oJ = (fJ & (kSgB - 1)) * kSgLd, oK = fK & (kSgB - 1);
-    const int tot = nj * nk;
-    const float rk = 1.0f / (float)nk;
-    for (int base = 0; base < tot; base += 64) {
-      const int idx = base + lane;
-      const bool ok = idx < tot;
-      // idx / nk: (idx + 0.5) / nk is >= 0.5 / 64 away from an integer, far beyond the
-      // float error at idx < 4096
-      int qa = (int)(((float)idx + 0.5f) * rk);
-      int qb = idx - qa * nk;
-      qa = ok ? qa : 0;
-      qb = ok ? qb : 0;
-      const double va = __shfl(aJ, qa, 64), vb = __shfl(aK, qb, 64);
-      const int o = __shfl(oJ, qa, 64) + __shfl(oK, qb, 64);
-      const double tt = fma(va, vb, kSgMagic);
-      const long long q = __double_as_longlong(tt) - __double_as_longlong(kSgMagic);
-      if (ok) atomicAdd(&acc[o], (unsigned long long)q);
-    }
-  };
-
   for (int64_t rb = r0 + (int64_t)w * 64; rb < r1; rb += (int64_t)kSgWaves * 64) {
     const int64_t i = rb + 4 * (lane & 15) + (lane >> 4);
     int64_t pj = 0, pk = 0;
@@ -342,42 +353,60 @@ __global__ void __launch_bounds__(kSgThreads) k_sg_tile(
       nk = k1 - k0;
     }
     const bool live = nj > 0 && nk > 0;
-    const bool small = live && nj <= 16 && nk <= 16;
-    uint64_t big = __ballot(live && !small);
+    const bool grouped = live && min(nj, nk) <= 16;
+    uint64_t single = __ballot(live && !grouped);
+    // an atomic instruction costs the same however many lanes it carries, so the longer
+    // segment that fits in 16 lanes goes to the lanes and the shorter one is broadcast
+    const bool swap = nj > 16 || (nk <= 16 && nk > nj);
     SgBatch bt;
-    bt.pj = pj;
-    bt.pk = pk;
-    bt.nj = small ? nj : 0;
-    bt.nk = small ? nk : 0;
-    int m = max(bt.nk, __shfl_xor(bt.nk, 16, 64));
-    bt.mk = max(m, __shfl_xor(m, 32, 64));
-    if (__ballot(small)) {
+    bt.pL = swap ? pk : pj;
+    bt.pB = swap ? pj : pk;
+    bt.nL = grouped ? (swap ? nk : nj) : 0;
+    bt.nB = grouped ? (swap ? nj : nk) : 0;
+    bt.mulL = swap ? kSgMulK : kSgMulJ;
+    bt.mulB = swap ? kSgMulJ : kSgMulK;
+    const int m = max(bt.nB, __shfl_xor(bt.nB, 16, 64));
+    bt.mB = max(m, __shfl_xor(m, 32, 64));
+    if (__ballot(grouped)) {
+      // 16 groups, one group's loads in flight while the previous one multiplies
       SgGroup ga, gb;
-      sg_fetch<0>(bt, ga, dv, fc, a);
-      sg_pipe<1>(acc, bt, ga, gb, dv, fc, a);
+      sg_fetch(bt, ga, dv, fc, a);
+      for (int G = 1;; G += 2) {
+        sg_fetch(bt, gb, dv, fc, a);
+        sg_group_products(acc, ga, dv, fc, a);
+        if (G + 1 == 16) {
+          sg_group_products(acc, gb, dv, fc, a);
+          break;
+        }
+        sg_fetch(bt, ga, dv, fc, a);
+        sg_group_products(acc, gb, dv, fc, a);
+      }
     }
 
-    // rows with more than 16 entries in a block: one row at a time, 64 x 64 chunks
-    while (big) {
-      const int q = __ffsll((unsigned long long)big) - 1;
-      big &= big - 1;
-      const int64_t sp = readlane_i64(pj, q), kp = readlane_i64(pk, q);
-      const int sn = __builtin_amdgcn_readlane(nj, q), kn = __builtin_amdgcn_readlane(nk, q);
-      for (int jc = 0; jc < sn; jc += 64) {
-        double vJ = 0.0;
-        int fJ = 0;
-        if (jc + lane < sn) {
-          vJ = dv[sp + jc + lane];
-          fJ = fc[sp + jc + lane];
-        }
-        for (int kc = 0; kc < kn; kc += 64) {
-          double vK = 0.0;
-          int fK = 0;
-          if (kc + lane < kn) {
-            vK = dv[kp + kc + lane];
-            fK = fc[kp + kc + lane];
-          }
-          products(vJ, fJ, vK, fK, min(64, sn - jc), min(64, kn - kc));
+    // rows with both segments longer than 16: one row at a time, the longer segment over
+    // the 64 lanes (chunks of 64), the shorter broadcast 16 entries at a time (each chunk
+    // loaded into all four 16-lane rows)
+    while (single) {
+      const int q = __ffsll((unsigned long long)single) - 1;
+      single &= single - 1;
+      const int64_t spj = readlane_i64(pj, q), spk = readlane_i64(pk, q);
+      const int snj = __builtin_amdgcn_readlane(nj, q), snk = __builtin_amdgcn_readlane(nk, q);
+      const bool jl = snj >= snk;
+      const int64_t sL = jl ? spj : spk, sB = jl ? spk : spj;
+      const int nL = jl ? snj : snk, nB = jl ? snk : snj;
+      const int mL = jl ? kSgMulJ : kSgMulK, mB = jl ? kSgMulK : kSgMulJ;
+      for (int lc = 0; lc < nL; lc += 64) {
+        const bool act = lc + lane < nL;
+        const int64_t iL = act ? sL + lc + lane : 0;
+        const double vL = dv[iL];
+        const int oL = (fc[iL] & (kSgB - 1)) * mL;
+        for (int c = 0; c < nB; c += 16) {
+          const bool ok = c + a < nB;
+          const int64_t iB = ok ? sB + c + a : 0;
+          const double vB = dv[iB];
+          const int fB = fc[iB];
+          sg_chunk(acc, vL, ok ? vB : 0.0, oL, (fB & (kSgB - 1)) * mB, act, 16,
+                   min(nB - c, 16));
         }
       }
     }
