@@ -10,8 +10,9 @@ with its loadings alone, ``data_coral @ L`` (``graph_datastore.py:886``, ``ann.p
 
 Here nothing of size cells x features is formed:
 
-* both covariances come from FP64 MFMA Grams of the standardised rows (``gram_of_rows``
-  with the query's or the reference's centering), turned into C = cov + I on the device
+* both covariances come from FP64 Grams of the standardised rows (``gram_of_rows`` with
+  the query's or the reference's centering: the sparse form with exact integer sums for
+  sparse counts, dense MFMA otherwise), turned into C = cov + I on the device
   (``sh_cov_from_gram``); the reference's is computed once per ``AnnStream``;
 * C_s^{-1/2} and C_t^{1/2} come from the coupled Newton-Schulz iteration on FP64 GEMMs
   (``sh_spd_sqrt``) instead of scipy's ``fractional_matrix_power``;

@@ -249,8 +249,9 @@ def chebyshev_subspace(dev, n_top: int, tol: float, dyn: float = 16.0,
 
     Each iteration filters the active block with a Chebyshev polynomial that damps
     [0, cut] (the merge matrix is positive semi-definite; cut is the block's smallest
-    Ritz value) and is scaled to 1 at an upper bound of the active spectrum, then
-    orthonormalises it (CholeskyQR2) and takes its Ritz pairs.  The degree is chosen so
+    Ritz value) and is scaled to 1 at an upper bound of the active spectrum, then takes
+    one CholeskyQR pass and the Ritz pairs of the generalized problem H q = theta G q (G the
+    block's Gram after that pass, so a second pass is not needed).  The degree is chosen so
     that the filtered block's condition number stays below e^dyn.  Leading pairs whose
     residual is far below the tolerance (so that removing them perturbs the rest by less
     than lock_factor * tol * theta_max) are locked and deflated out of the working
