@@ -197,9 +197,9 @@ __device__ __forceinline__ int sg_bcast32(int v) {
 // segment can take either role.
 constexpr int kSgMulJ = kSgLd * 8, kSgMulK = 8;
 
-// A wave's batch of 64 rows, laid out so that group G (0..15) is rows 4G..4G+3: lane L
-// holds row 4 (L % 16) + L / 16, so the four rows of group G sit in lane G of each 16-lane
-// slot.  After each group the batch rotates by one lane within the slots, so the next
+// A wave's batch of 64 rows, ranked by broadcast length and laid out so that group G
+// (0..15) is ranks 4G..4G+3: lane L holds rank 4 (L % 16) + L / 16, so the four rows of
+// group G sit in lane G of each 16-lane slot.  After each group the batch rotates by one lane within the slots, so the next
 // group's rows are always in lane 0 and one row_newbcast:0 hands each slot its row's
 // segments.  Rows outside the group path (both segments longer than 16, or one empty)
 // carry nL = nB = 0 here.
@@ -340,7 +340,7 @@ __global__ void __launch_bounds__(kSgThreads) k_sg_tile(
   const int a = lane & 15;
 
   for (int64_t rb = r0 + (int64_t)w * 64; rb < r1; rb += (int64_t)kSgWaves * 64) {
-    const int64_t i = rb + 4 * (lane & 15) + (lane >> 4);
+    const int64_t i = rb + lane;
     int64_t pj = 0, pk = 0;
     int nj = 0, nk = 0;
     if (i < r1) {
@@ -358,13 +358,31 @@ __global__ void __launch_bounds__(kSgThreads) k_sg_tile(
     // an atomic instruction costs the same however many lanes it carries, so the longer
     // segment that fits in 16 lanes goes to the lanes and the shorter one is broadcast
     const bool swap = nj > 16 || (nk <= 16 && nk > nj);
+    const int64_t rpL = swap ? pk : pj, rpB = swap ? pj : pk;
+    const int rnL = grouped ? (swap ? nk : nj) : 0, rnB = grouped ? (swap ? nj : nk) : 0;
+    const int rmL = swap ? kSgMulK : kSgMulJ, rmB = swap ? kSgMulJ : kSgMulK;
+    // a group takes as many steps as its longest broadcast segment: rank the batch's rows
+    // by broadcast length (bitonic sort of (127 - nB, lane) across the wave) so that each
+    // group of four holds rows of similar length, rows outside the group path last
+    uint32_t key = ((uint32_t)(127 - min(rnB, 127)) << 6) | (uint32_t)lane;
+#pragma unroll
+    for (int kk = 2; kk <= 64; kk <<= 1) {
+#pragma unroll
+      for (int jj = kk >> 1; jj > 0; jj >>= 1) {
+        const uint32_t o = __shfl_xor(key, jj, 64);
+        const bool up = (lane & kk) == 0, lower = (lane & jj) == 0;
+        key = (lower == up) ? min(key, o) : max(key, o);
+      }
+    }
+    // lane 16 s + G takes the row of rank 4 G + s
+    const int src = (int)(__shfl(key, 4 * (lane & 15) + (lane >> 4), 64) & 63u);
     SgBatch bt;
-    bt.pL = swap ? pk : pj;
-    bt.pB = swap ? pj : pk;
-    bt.nL = grouped ? (swap ? nk : nj) : 0;
-    bt.nB = grouped ? (swap ? nj : nk) : 0;
-    bt.mulL = swap ? kSgMulK : kSgMulJ;
-    bt.mulB = swap ? kSgMulJ : kSgMulK;
+    bt.pL = __shfl(rpL, src, 64);
+    bt.pB = __shfl(rpB, src, 64);
+    bt.nL = __shfl(rnL, src, 64);
+    bt.nB = __shfl(rnB, src, 64);
+    bt.mulL = __shfl(rmL, src, 64);
+    bt.mulB = __shfl(rmB, src, 64);
     const int m = max(bt.nB, __shfl_xor(bt.nB, 16, 64));
     bt.mB = max(m, __shfl_xor(m, 32, 64));
     if (__ballot(grouped)) {
