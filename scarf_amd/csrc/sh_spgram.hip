@@ -21,14 +21,18 @@
 // Kernels.
 //   k_sg_len + hipcub scan: packed-row offsets P (row length + 1 for a ones entry).
 //   k_sg_pack: one wave per row; entries bucketed by 128-feature block (a counting sort in
-//     wave-private LDS), delta in FP64 and the 7-bit column within its block; per-row block
-//     offsets BO; column maxima of |delta| (float, rounded up).  Feature d is a ones column
-//     (delta = 1): its products with feature f give s_f.
+//     wave-private LDS), delta in FP64 and the feature as u16; per-row block offsets BO;
+//     column maxima of |delta| (float, rounded up).  Feature d is a ones column (delta = 1):
+//     its products with feature f give s_f.
+//   k_sg_scale: every packed delta times 2^(F/2 - e_f), once the maxima are known.
 //   k_sg_tile: one 1024-thread workgroup per (128 x 128 block tile of the upper triangle, row
-//     slab); the tile accumulates in LDS as int64 [128][129]; each wave takes 64 rows at a
-//     time (their block-segment bounds in one load per lane), prefetches the next row's two
-//     segments while it forms the current row's nJ x nK products, flattened over the 64
-//     lanes, and adds them with ds_add_u64.  Slab tiles go to global memory as int64.
+//     slab); the tile accumulates in LDS as int64 [128][129].  Each wave takes 64 rows at a
+//     time, ranks them by the length of their broadcast segment and multiplies them four
+//     at a time: each 16-lane slot holds one row's shorter (at most 16-entry) segment, and
+//     each step broadcasts one entry of the other segment to its slot with DPP
+//     row_newbcast, so one ds_add_u64 carries up to 64 products.  The next group's loads
+//     are in flight while a group multiplies.  Rows with more than 16 entries in both
+//     segments go one at a time over the 64 lanes.  Slab tiles go to memory as int64.
 //   k_sg_colsum / k_sg_finish: slab tiles summed in FP64 in slab order, scales applied and the
 //     rank-one terms added: G (full symmetric) and the column sums.
 #include <hipcub/hipcub.hpp>
