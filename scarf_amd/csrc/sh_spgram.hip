@@ -59,7 +59,9 @@ __device__ __forceinline__ int sg_col_exp(uint32_t bits) {
   (void)frexpf(v, &e);  // v = m 2^e, m in [0.5, 1)
   return e;
 }
-__device__ __forceinline__ bool sg_finite(uint32_t bits) { return (bits & 0x7f800000u) != 0x7f800000u; }
+__device__ __forceinline__ bool sg_finite(uint32_t bits) {
+  return (bits & 0x7f800000u) != 0x7f800000u;
+}
 
 // z of a zero count for feature f.
 __device__ __forceinline__ double sg_z0(const double *center, const double *scale, double clip,
@@ -203,10 +205,10 @@ constexpr int kSgMulJ = kSgLd * 8, kSgMulK = 8;
 
 // A wave's batch of 64 rows, ranked by broadcast length and laid out so that group G
 // (0..15) is ranks 4G..4G+3: lane L holds rank 4 (L % 16) + L / 16, so the four rows of
-// group G sit in lane G of each 16-lane slot.  After each group the batch rotates by one lane within the slots, so the next
-// group's rows are always in lane 0 and one row_newbcast:0 hands each slot its row's
-// segments.  Rows outside the group path (both segments longer than 16, or one empty)
-// carry nL = nB = 0 here.
+// group G sit in lane G of each 16-lane slot.  After each group the batch rotates by one
+// lane within the slots, so the next group's rows are always in lane 0 and one
+// row_newbcast:0 hands each slot its row's segments.  Rows outside the group path (both
+// segments longer than 16, or one empty) carry nL = nB = 0 here.
 struct SgBatch {
   int64_t pL, pB;
   int nL, nB, mulL, mulB;
