@@ -136,3 +136,34 @@ def test_sparse_gram_bit_reproducible_and_accumulates(cuda):
     empty = torch.zeros(0, device="cuda", dtype=torch.int64)
     G0, c0, m0 = gram_of_rows(dev, st, p, empty, method="sparse")
     assert m0 == 0 and not G0.any() and not c0.any()
+
+
+def test_gram_method_choice(cuda):
+    """method="auto" takes the dense MFMA form above 30% density and the sparse one below;
+    both agree with the host Gram.  The sparse form refuses more than 8063 features."""
+    import torch
+    from scarf_amd.device import DeviceCSR
+    from scarf_amd.normalize import NormParams, feature_stats
+    from scarf_amd.pca import SPARSE_GRAM_MAX_D, gram_of_rows
+    rng = np.random.default_rng(5)
+    dense = rng.poisson(1.5, (1500, 200)).astype(np.uint32)  # ~78% non-zero
+    csr = CSR.from_dense(dense)
+    dev = DeviceCSR.from_host(csr)
+    p = NormParams()
+    st = feature_stats(dev, 500, p)
+    rows = torch.arange(1500, device="cuda", dtype=torch.int64)
+    say("gram method choice, density", csr.vals.shape[0] / dense.size)
+    G_auto, cs_auto, _ = gram_of_rows(dev, st, p, rows)
+    G_dense, cs_dense, _ = gram_of_rows(dev, st, p, rows, method="dense")
+    G_sparse, _, _ = gram_of_rows(dev, st, p, rows, method="sparse")
+    assert np.array_equal(G_auto, G_dense) and np.array_equal(cs_auto, cs_dense)
+    Gr, _, scale = _host_gram(dense, st, p, None, np.arange(1500))
+    den = np.maximum(scale, 1e-300)
+    assert (np.abs(G_sparse - Gr) / den).max() < 1e-12
+    assert (np.abs(G_dense - Gr) / den).max() < 1e-12
+    wide = CSR.from_dense(np.eye(4, SPARSE_GRAM_MAX_D + 1, dtype=np.uint32))
+    dw = DeviceCSR.from_host(wide)
+    stw = feature_stats(dw, 4, p)
+    with pytest.raises(ValueError, match="features"):
+        gram_of_rows(dw, stw, p, torch.arange(4, device="cuda", dtype=torch.int64),
+                     method="sparse")
