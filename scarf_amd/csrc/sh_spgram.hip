@@ -31,8 +31,9 @@
 //     at a time: each 16-lane slot holds one row's shorter (at most 16-entry) segment, and
 //     each step broadcasts one entry of the other segment to its slot with DPP
 //     row_newbcast, so one ds_add_u64 carries up to 64 products.  The next group's loads
-//     are in flight while a group multiplies.  Rows with more than 16 entries in both
-//     segments go one at a time over the 64 lanes.  Slab tiles go to memory as int64.
+//     are in flight while a group multiplies.  A row with more than 16 entries in both
+//     segments is cut into 16-entry pieces of its longer segment, which go through the
+//     same groups.  Slab tiles go to memory as int64.
 //   k_sg_colsum / k_sg_finish: slab tiles summed in FP64 in slab order, scales applied and the
 //     rank-one terms added: G (full symmetric) and the column sums.
 #include <hipcub/hipcub.hpp>
@@ -319,6 +320,59 @@ __device__ __forceinline__ void sg_group_products(unsigned long long *acc, SgGro
   }
 }
 
+// Multiplies a wave's batch of up to 64 rows given in natural lane order (lane = row; nL = 0
+// for a lane without a row; every row's lane segment holds at most 16 entries): ranks the
+// rows by broadcast length (bitonic sort of (127 - nB, lane) across the wave) so that each
+// group of four holds rows of similar length (a group takes as many steps as its longest
+// broadcast segment), lays them out by group and runs the groups that hold rows with one
+// group's loads in flight while the previous one multiplies.
+__device__ __forceinline__ void sg_run_batch(unsigned long long *acc, int64_t rpL, int64_t rpB,
+                                             int rnL, int rnB, int rmL, int rmB,
+                                             const double *__restrict__ dv,
+                                             const uint16_t *__restrict__ fc, int lane) {
+  const int a = lane & 15;
+  uint32_t key = ((uint32_t)(127 - min(rnL > 0 ? rnB : 0, 127)) << 6) | (uint32_t)lane;
+#pragma unroll
+  for (int kk = 2; kk <= 64; kk <<= 1) {
+#pragma unroll
+    for (int jj = kk >> 1; jj > 0; jj >>= 1) {
+      const uint32_t o = __shfl_xor(key, jj, 64);
+      const bool up = (lane & kk) == 0, lower = (lane & jj) == 0;
+      key = (lower == up) ? min(key, o) : max(key, o);
+    }
+  }
+  // lane 16 s + G takes the row of rank 4 G + s
+  const int src = (int)(__shfl(key, 4 * (lane & 15) + (lane >> 4), 64) & 63u);
+  SgBatch bt;
+  bt.pL = __shfl(rpL, src, 64);
+  bt.pB = __shfl(rpB, src, 64);
+  bt.nL = __shfl(rnL, src, 64);
+  bt.nB = __shfl(rnL > 0 ? rnB : 0, src, 64);
+  bt.mulL = __shfl(rmL, src, 64);
+  bt.mulB = __shfl(rmB, src, 64);
+  const int m = max(bt.nB, __shfl_xor(bt.nB, 16, 64));
+  bt.mB = max(m, __shfl_xor(m, 32, 64));
+  // rows are ranked first, empty lanes last: only the first ng groups hold rows
+  const int ng = (int)((__popcll(__ballot(rnL > 0)) + 3) >> 2);
+  if (ng == 0) return;
+  SgGroup ga, gb;
+  sg_fetch(bt, ga, dv, fc, a);
+  for (int G = 1;; G += 2) {
+    if (G == ng) {
+      sg_group_products(acc, ga, dv, fc, a);
+      break;
+    }
+    sg_fetch(bt, gb, dv, fc, a);
+    sg_group_products(acc, ga, dv, fc, a);
+    if (G + 1 == ng) {
+      sg_group_products(acc, gb, dv, fc, a);
+      break;
+    }
+    sg_fetch(bt, ga, dv, fc, a);
+    sg_group_products(acc, gb, dv, fc, a);
+  }
+}
+
 __global__ void __launch_bounds__(kSgThreads) k_sg_tile(
     const int64_t *__restrict__ P, const int32_t *__restrict__ BO,
     const double *__restrict__ dv, const uint16_t *__restrict__ fc, int64_t n, int nb, int T,
@@ -343,7 +397,6 @@ __global__ void __launch_bounds__(kSgThreads) k_sg_tile(
   const int nb1 = nb + 1;
   const int64_t r0 = (int64_t)slab * rows_per_slab;
   const int64_t r1 = min(n, r0 + rows_per_slab);
-  const int a = lane & 15;
 
   for (int64_t rb = r0 + (int64_t)w * 64; rb < r1; rb += (int64_t)kSgWaves * 64) {
     const int64_t i = rb + lane;
@@ -364,52 +417,15 @@ __global__ void __launch_bounds__(kSgThreads) k_sg_tile(
     // an atomic instruction costs the same however many lanes it carries, so the longer
     // segment that fits in 16 lanes goes to the lanes and the shorter one is broadcast
     const bool swap = nj > 16 || (nk <= 16 && nk > nj);
-    const int64_t rpL = swap ? pk : pj, rpB = swap ? pj : pk;
-    const int rnL = grouped ? (swap ? nk : nj) : 0, rnB = grouped ? (swap ? nj : nk) : 0;
-    const int rmL = swap ? kSgMulK : kSgMulJ, rmB = swap ? kSgMulJ : kSgMulK;
-    // a group takes as many steps as its longest broadcast segment: rank the batch's rows
-    // by broadcast length (bitonic sort of (127 - nB, lane) across the wave) so that each
-    // group of four holds rows of similar length, rows outside the group path last
-    uint32_t key = ((uint32_t)(127 - min(rnB, 127)) << 6) | (uint32_t)lane;
-#pragma unroll
-    for (int kk = 2; kk <= 64; kk <<= 1) {
-#pragma unroll
-      for (int jj = kk >> 1; jj > 0; jj >>= 1) {
-        const uint32_t o = __shfl_xor(key, jj, 64);
-        const bool up = (lane & kk) == 0, lower = (lane & jj) == 0;
-        key = (lower == up) ? min(key, o) : max(key, o);
-      }
-    }
-    // lane 16 s + G takes the row of rank 4 G + s
-    const int src = (int)(__shfl(key, 4 * (lane & 15) + (lane >> 4), 64) & 63u);
-    SgBatch bt;
-    bt.pL = __shfl(rpL, src, 64);
-    bt.pB = __shfl(rpB, src, 64);
-    bt.nL = __shfl(rnL, src, 64);
-    bt.nB = __shfl(rnB, src, 64);
-    bt.mulL = __shfl(rmL, src, 64);
-    bt.mulB = __shfl(rmB, src, 64);
-    const int m = max(bt.nB, __shfl_xor(bt.nB, 16, 64));
-    bt.mB = max(m, __shfl_xor(m, 32, 64));
-    if (__ballot(grouped)) {
-      // 16 groups, one group's loads in flight while the previous one multiplies
-      SgGroup ga, gb;
-      sg_fetch(bt, ga, dv, fc, a);
-      for (int G = 1;; G += 2) {
-        sg_fetch(bt, gb, dv, fc, a);
-        sg_group_products(acc, ga, dv, fc, a);
-        if (G + 1 == 16) {
-          sg_group_products(acc, gb, dv, fc, a);
-          break;
-        }
-        sg_fetch(bt, ga, dv, fc, a);
-        sg_group_products(acc, gb, dv, fc, a);
-      }
-    }
-
-    // rows with both segments longer than 16: one row at a time, the longer segment over
-    // the 64 lanes (chunks of 64), the shorter broadcast 16 entries at a time (each chunk
-    // loaded into all four 16-lane rows)
+    if (__ballot(grouped))
+      sg_run_batch(acc, swap ? pk : pj, swap ? pj : pk, grouped ? (swap ? nk : nj) : 0,
+                   swap ? nj : nk, swap ? kSgMulK : kSgMulJ, swap ? kSgMulJ : kSgMulK, dv, fc,
+                   lane);
+    // rows with both segments longer than 16: the longer segment is cut into 16-entry
+    // pieces, each a row of its own with the whole shorter segment broadcast, and those
+    // rows go through the same ranked groups (64 at a time)
+    int64_t vpL = 0, vpB = 0;
+    int vnL = 0, vnB = 0, vmL = 0, vmB = 0, nv = 0;
     while (single) {
       const int q = __ffsll((unsigned long long)single) - 1;
       single &= single - 1;
@@ -419,21 +435,24 @@ __global__ void __launch_bounds__(kSgThreads) k_sg_tile(
       const int64_t sL = jl ? spj : spk, sB = jl ? spk : spj;
       const int nL = jl ? snj : snk, nB = jl ? snk : snj;
       const int mL = jl ? kSgMulJ : kSgMulK, mB = jl ? kSgMulK : kSgMulJ;
-      for (int lc = 0; lc < nL; lc += 64) {
-        const bool act = lc + lane < nL;
-        const int64_t iL = act ? sL + lc + lane : 0;
-        const double vL = dv[iL];
-        const int oL = (fc[iL] & (kSgB - 1)) * mL;
-        for (int c = 0; c < nB; c += 16) {
-          const bool ok = c + a < nB;
-          const int64_t iB = ok ? sB + c + a : 0;
-          const double vB = dv[iB];
-          const int fB = fc[iB];
-          sg_chunk(acc, vL, ok ? vB : 0.0, oL, (fB & (kSgB - 1)) * mB, act, 16,
-                   min(nB - c, 16));
+      for (int c = 0; c < nL; c += 16) {
+        if (nv == 64) {
+          sg_run_batch(acc, vpL, vpB, vnL, vnB, vmL, vmB, dv, fc, lane);
+          vnL = 0;
+          nv = 0;
         }
+        if (lane == nv) {
+          vpL = sL + c;
+          vnL = min(16, nL - c);
+          vpB = sB;
+          vnB = nB;
+          vmL = mL;
+          vmB = mB;
+        }
+        ++nv;
       }
     }
+    if (nv) sg_run_batch(acc, vpL, vpB, vnL, vnB, vmL, vmB, dv, fc, lane);
   }
   __syncthreads();
   long long *dst = part + ((int64_t)slab * T + t) * kSgTileElems;
