@@ -44,16 +44,30 @@ __global__ void k_order_minmax(const float *__restrict__ E, int64_t n, int32_t D
       }
     }
   }
+  // wave, then workgroup (LDS), then one atomic per workgroup and coordinate: the six
+  // words are shared by every workgroup, so per-wave atomics would queue on them
+  __shared__ uint32_t red[4][2 * kOrderCoords];
+  const int w = threadIdx.x >> 6;
 #pragma unroll
   for (int c = 0; c < kOrderCoords; ++c) {
     for (int m = 32; m >= 1; m >>= 1) {
       lo[c] = min(lo[c], (uint32_t)__shfl_xor((int)lo[c], m, kWave));
       hi[c] = max(hi[c], (uint32_t)__shfl_xor((int)hi[c], m, kWave));
     }
-    if ((threadIdx.x & 63) == 0 && c < nc) {
-      atomicMin(&mm[2 * c], lo[c]);
-      atomicMax(&mm[2 * c + 1], hi[c]);
+    if ((threadIdx.x & 63) == 0) {
+      red[w][2 * c] = lo[c];
+      red[w][2 * c + 1] = hi[c];
     }
+  }
+  __syncthreads();
+  if (threadIdx.x < 2 * kOrderCoords && (int)threadIdx.x < 2 * nc) {
+    const int j = threadIdx.x;
+    uint32_t v = red[0][j];
+    for (int i = 1; i < 4; ++i) v = (j & 1) ? max(v, red[i][j]) : min(v, red[i][j]);
+    if (j & 1)
+      atomicMax(&mm[j], v);
+    else
+      atomicMin(&mm[j], v);
   }
 }
 
@@ -137,7 +151,7 @@ extern "C" int sh_locality_order(const float *E, int64_t n, int32_t D, int64_t l
   size_t tmp_bytes = sort_temp_bytes(n);
   void *tmp = p;
   hipLaunchKernelGGL(k_order_init, dim3(1), dim3(64), 0, st, mm);
-  const int64_t blocks = std::min<int64_t>(cdiv(n, 256), 2048);
+  const int64_t blocks = std::min<int64_t>(cdiv(n, 256), 512);
   hipLaunchKernelGGL(k_order_minmax, dim3((uint32_t)blocks), dim3(256), 0, st, E, n, D, lde, nc,
                      mm);
   hipLaunchKernelGGL(k_order_keys, dim3((uint32_t)cdiv(n, 256)), dim3(256), 0, st, E, n, lde,

@@ -19,22 +19,43 @@ namespace sh {
 constexpr float kSmoothTol = 1e-5f;
 constexpr float kMinKDistScale = 1e-3f;
 
-// Deterministic FP64 mean of each chunk's n_c * k distances (fixed tree order).
-__global__ void __launch_bounds__(256) k_chunk_mean(const float *__restrict__ dist, int64_t n,
-                                                    int32_t k, int64_t cs,
-                                                    double *__restrict__ mean_out) {
+// Deterministic FP64 mean of each chunk's n_c * k distances.  A chunk's entries are cut
+// into mean_parts(chunk_size) contiguous parts, one workgroup each (20 chunks at 1M rows
+// would otherwise leave the GPU nearly idle); each part sums in a fixed tree order and the
+// parts are added in order, so the result depends only on the inputs.
+constexpr int kMeanParts = 64;
+inline int64_t mean_parts(int64_t cs) { return std::min<int64_t>(kMeanParts, cdiv(cs, 128)); }
+
+__global__ void __launch_bounds__(256) k_chunk_mean_part(const float *__restrict__ dist,
+                                                         int64_t n, int32_t k, int64_t cs,
+                                                         int32_t parts,
+                                                         double *__restrict__ part_out) {
   __shared__ double red[256];
   const int64_t c = blockIdx.x;
   const int64_t a = c * cs * k, b = min(n, (c + 1) * cs) * (int64_t)k;
+  const int64_t len = (b - a + parts - 1) / parts;
+  const int64_t pa = min(b, a + (int64_t)blockIdx.y * len), pb = min(b, pa + len);
   double s = 0.0;
-  for (int64_t t = a + threadIdx.x; t < b; t += 256) s += (double)dist[t];
+  for (int64_t t = pa + threadIdx.x; t < pb; t += 256) s += (double)dist[t];
   red[threadIdx.x] = s;
   __syncthreads();
   for (int o = 128; o > 0; o >>= 1) {
     if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
     __syncthreads();
   }
-  if (threadIdx.x == 0) mean_out[c] = b > a ? red[0] / (double)(b - a) : 0.0;
+  if (threadIdx.x == 0) part_out[c * parts + blockIdx.y] = red[0];
+}
+
+__global__ void __launch_bounds__(256) k_chunk_mean_sum(const double *__restrict__ part,
+                                                        int64_t n, int32_t k, int64_t cs,
+                                                        int32_t parts, int64_t nc,
+                                                        double *__restrict__ mean_out) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  const int64_t cnt = (min(n, (c + 1) * cs) - c * cs) * (int64_t)k;
+  double s = 0.0;
+  for (int p = 0; p < parts; ++p) s += part[c * parts + p];
+  mean_out[c] = cnt > 0 ? s / (double)cnt : 0.0;
 }
 
 template <int KM>
@@ -201,7 +222,8 @@ extern "C" size_t sh_smooth_knn_workspace(int64_t n, int64_t chunk_size) {
   if (chunk_size <= 0) return 0;
   const int64_t nc = cdiv(n, chunk_size);
   return align_up(nc * sizeof(double), 256) + align_up(nc * sizeof(uint32_t), 256) +
-         align_up(nc * sizeof(int32_t), 256) + 256;
+         align_up(nc * sizeof(int32_t), 256) + 256 +
+         align_up(nc * mean_parts(chunk_size) * sizeof(double), 256);
 }
 
 extern "C" int sh_smooth_knn(const float *dist, const int64_t *idx, int64_t n, int32_t k,
@@ -227,10 +249,17 @@ extern "C" int sh_smooth_knn(const float *dist, const int64_t *idx, int64_t n, i
   int32_t *czero = (int32_t *)p;
   p += align_up(nc * sizeof(int32_t), 256);
   float *gmin = (float *)p;
+  p += 256;
+  double *cpart = (double *)p;
   hipLaunchKernelGGL(k_smooth_init, dim3(cdiv(nc, 256)), dim3(256), 0, st, cmin, czero, nc);
   SH_LAUNCH_CHECK("k_smooth_init");
-  hipLaunchKernelGGL(k_chunk_mean, dim3(nc), dim3(256), 0, st, dist, n, k, chunk_size, cmean);
-  SH_LAUNCH_CHECK("k_chunk_mean");
+  const int32_t parts = (int32_t)mean_parts(chunk_size);
+  hipLaunchKernelGGL(k_chunk_mean_part, dim3(nc, parts), dim3(256), 0, st, dist, n, k,
+                     chunk_size, parts, cpart);
+  SH_LAUNCH_CHECK("k_chunk_mean_part");
+  hipLaunchKernelGGL(k_chunk_mean_sum, dim3(cdiv(nc, 256)), dim3(256), 0, st, cpart, n, k,
+                     chunk_size, parts, nc, cmean);
+  SH_LAUNCH_CHECK("k_chunk_mean_sum");
   dim3 grid(cdiv(n, 256)), block(256);
 #define SH_SMOOTH(KM)                                                                           \
   hipLaunchKernelGGL(k_smooth_rows<KM>, grid, block, 0, st, dist, idx, n, k, chunk_size,        \
