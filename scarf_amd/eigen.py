@@ -115,8 +115,9 @@ def top_eigh_device(M: "torch.Tensor", n_top: int, tol: float = 1e-12,
     if not 1 <= n_top <= d:
         raise ValueError(f"n_top={n_top} must be in [1, {d}]")
     keep, kin = restart_sizes(n_top, d, max_vectors)
-    dev = _DeviceLanczos(M, kin, seed, stream)
-    return thick_restart_lanczos(dev, n_top, keep, kin, tol)
+    with torch.cuda.stream(stream):  # read-backs on the kernels' stream
+        dev = _DeviceLanczos(M, kin, seed, stream)
+        return thick_restart_lanczos(dev, n_top, keep, kin, tol)
 
 
 def restart_sizes(n_top: int, d: int, max_vectors: Optional[int] = None):
@@ -350,8 +351,11 @@ def top_eigh_chebyshev(M: "torch.Tensor", n_top: int, tol: float = 1e-12, seed: 
     b = chebyshev_block_size(n_top, d) if block is None else int(block)
     if b <= n_top or b > 256 or b > d:
         raise EigenNotConverged(f"block {b} cannot hold {n_top} pairs plus guards (d={d})")
-    dev = _DeviceBlock(M, n_top, b, seed, stream, start)
-    return chebyshev_subspace(dev, n_top, tol)
+    # torch's copies and read-backs on the stream the library kernels use (a no-op context
+    # for the current stream)
+    with torch.cuda.stream(stream):
+        dev = _DeviceBlock(M, n_top, b, seed, stream, start)
+        return chebyshev_subspace(dev, n_top, tol)
 
 
 _RANDOM_BLOCKS = {}
