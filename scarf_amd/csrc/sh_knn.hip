@@ -23,6 +23,8 @@
 // queries in locality order: the block's own rows) and starts tau at their pool-th key.
 #include <float.h>
 
+#include <type_traits>
+
 #include "sh_common.h"
 #include "sh_wave_sort.h"
 
@@ -238,9 +240,12 @@ struct ScreenShape {
 };
 // Register budget of one instantiation: three workgroups per CU (168 VGPRs) unless the
 // query fragments or the compaction sort need more -- the FP8 screen at K >= 128 and the
-// 1024-key buffers -- which then run two per CU instead of spilling to scratch.
+// 1024-key buffers -- which then run two per CU instead of spilling to scratch.  The BF16
+// screen at K <= 64 with 256-key buffers runs four per CU (128 VGPRs; its spills fall
+// outside the tile loop): 166.6 against 174.6 ms at 1M x 1M, alternated runs on one GPU.
 template <class Op, int K, int CAP>
 constexpr int screen_minb() {
+  if constexpr (std::is_same<Op, OpBF16>::value && K <= 64 && CAP <= 256) return 4;
   return (Op::kScaled && K >= 128) || CAP >= 1024 ? 2 : ScreenShape<Op, K>::MINB;
 }
 
@@ -289,13 +294,22 @@ __device__ __forceinline__ bool wave_any(bool p) { return __builtin_amdgcn_ballo
 
 // Appends this lane's values of one 4-reference quad that pass tau to its buffer half.
 // Queries past the batch end carry tau = -inf, so nothing of theirs passes.
-template <int V0>
+// LATE: the reference index is added inside the store (an asm statement the compiler
+// cannot hoist).  Hoisted, a tile that takes this path computes all 16 indices and keeps
+// them in registers, which the four-workgroups-per-CU budget cannot spare; at three per CU
+// the hoisted form is faster.
+template <int V0, bool LATE>
 __device__ __forceinline__ void append_quad(const floatx16 &acc, float tau, uint32_t ref0,
                                             uint64_t *half, int &cnt) {
 #pragma unroll
   for (int v = V0; v < V0 + 4; ++v) {
     if (acc[v] <= tau) {
-      half[cnt] = make_key(acc[v], ref0 + (uint32_t)(8 * (v >> 2) + (v & 3)));
+      uint32_t ref;
+      if constexpr (LATE)
+        asm volatile("v_add_u32 %0, %1, %2" : "=v"(ref) : "i"(8 * (v >> 2) + (v & 3)), "v"(ref0));
+      else
+        ref = ref0 + (uint32_t)(8 * (v >> 2) + (v & 3));
+      half[cnt] = make_key(acc[v], ref);
       ++cnt;
     }
   }
@@ -308,6 +322,7 @@ __global__ void __launch_bounds__(256, (screen_minb<Op, K, CAP>())) k_screen(
     uint64_t *__restrict__ buf, uint64_t *__restrict__ out, int32_t *__restrict__ out_cnt) {
   typedef typename Op::Frag Frag;
   constexpr int QG = ScreenShape<Op, K>::QG;
+  constexpr bool kLate = screen_minb<Op, K, CAP>() == 4;  // see append_quad
   constexpr int QPW = 32 * QG;                // queries per wave
   constexpr int QPB = ScreenShape<Op, K>::QPB;
   constexpr int HALFC = CAP / 2;              // keys per buffer half
@@ -480,10 +495,10 @@ __global__ void __launch_bounds__(256, (screen_minb<Op, K, CAP>())) k_screen(
             if ((badw >> (8 * (v >> 2) + 4 * h + (v & 3))) & 1u) acc[g][v] = INFINITY;
           const int64_t q = qb0 + qw0 + g * 32 + n;
           uint64_t *half = buf + ((qvalid[g] ? q : 0) * S + slice) * CAP + h * HALFC;
-          append_quad<0>(acc[g], tau[g], 0u, half, cnt[g]);
-          append_quad<4>(acc[g], tau[g], 0u, half, cnt[g]);
-          append_quad<8>(acc[g], tau[g], 0u, half, cnt[g]);
-          append_quad<12>(acc[g], tau[g], 0u, half, cnt[g]);
+          append_quad<0, kLate>(acc[g], tau[g], 0u, half, cnt[g]);
+          append_quad<4, kLate>(acc[g], tau[g], 0u, half, cnt[g]);
+          append_quad<8, kLate>(acc[g], tau[g], 0u, half, cnt[g]);
+          append_quad<12, kLate>(acc[g], tau[g], 0u, half, cnt[g]);
         }
       }
       __syncthreads();  // the next stage overwrites the buffer
@@ -586,10 +601,14 @@ __global__ void __launch_bounds__(256, (screen_minb<Op, K, CAP>())) k_screen(
           if (!wave_any(hit[g])) continue;
           const int64_t q = qb0 + qw0 + g * 32 + n;
           uint64_t *half = buf + ((qvalid[g] ? q : 0) * S + slice) * CAP + h * HALFC;
-          if (wave_any(m4[g][0] <= tau[g])) append_quad<0>(acc[g], tau[g], ref0, half, cnt[g]);
-          if (wave_any(m4[g][1] <= tau[g])) append_quad<4>(acc[g], tau[g], ref0, half, cnt[g]);
-          if (wave_any(m4[g][2] <= tau[g])) append_quad<8>(acc[g], tau[g], ref0, half, cnt[g]);
-          if (wave_any(m4[g][3] <= tau[g])) append_quad<12>(acc[g], tau[g], ref0, half, cnt[g]);
+          if (wave_any(m4[g][0] <= tau[g]))
+            append_quad<0, kLate>(acc[g], tau[g], ref0, half, cnt[g]);
+          if (wave_any(m4[g][1] <= tau[g]))
+            append_quad<4, kLate>(acc[g], tau[g], ref0, half, cnt[g]);
+          if (wave_any(m4[g][2] <= tau[g]))
+            append_quad<8, kLate>(acc[g], tau[g], ref0, half, cnt[g]);
+          if (wave_any(m4[g][3] <= tau[g]))
+            append_quad<12, kLate>(acc[g], tau[g], ref0, half, cnt[g]);
           // compact every query of the group one of whose halves cannot take a tile
           const uint64_t mk = __ballot(cnt[g] > HALFC - 16);
           uint32_t mq = (uint32_t)(mk | (mk >> 32));
