@@ -298,21 +298,28 @@ __device__ __forceinline__ bool wave_any(bool p) { return __builtin_amdgcn_ballo
 // cannot hoist).  Hoisted, a tile that takes this path computes all 16 indices and keeps
 // them in registers, which the four-workgroups-per-CU budget cannot spare; at three per CU
 // the hoisted form is faster.
+template <int V, bool LATE>
+__device__ __forceinline__ void append_one(const floatx16 &acc, float tau, uint32_t ref0,
+                                           uint64_t *half, int &cnt) {
+  constexpr int kOff = 8 * (V >> 2) + (V & 3);  // the value's reference within the tile
+  if (acc[V] <= tau) {
+    uint32_t ref;
+    if constexpr (LATE)
+      asm volatile("v_add_u32 %0, %1, %2" : "=v"(ref) : "i"(kOff), "v"(ref0));
+    else
+      ref = ref0 + (uint32_t)kOff;
+    half[cnt] = make_key(acc[V], ref);
+    ++cnt;
+  }
+}
+
 template <int V0, bool LATE>
 __device__ __forceinline__ void append_quad(const floatx16 &acc, float tau, uint32_t ref0,
                                             uint64_t *half, int &cnt) {
-#pragma unroll
-  for (int v = V0; v < V0 + 4; ++v) {
-    if (acc[v] <= tau) {
-      uint32_t ref;
-      if constexpr (LATE)
-        asm volatile("v_add_u32 %0, %1, %2" : "=v"(ref) : "i"(8 * (v >> 2) + (v & 3)), "v"(ref0));
-      else
-        ref = ref0 + (uint32_t)(8 * (v >> 2) + (v & 3));
-      half[cnt] = make_key(acc[v], ref);
-      ++cnt;
-    }
-  }
+  append_one<V0, LATE>(acc, tau, ref0, half, cnt);
+  append_one<V0 + 1, LATE>(acc, tau, ref0, half, cnt);
+  append_one<V0 + 2, LATE>(acc, tau, ref0, half, cnt);
+  append_one<V0 + 3, LATE>(acc, tau, ref0, half, cnt);
 }
 
 template <class Op, int K, int CAP>
