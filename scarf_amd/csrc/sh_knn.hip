@@ -243,9 +243,17 @@ struct ScreenShape {
 // 1024-key buffers -- which then run two per CU instead of spilling to scratch.  The BF16
 // screen at K <= 64 with 256-key buffers runs four per CU (128 VGPRs; its spills fall
 // outside the tile loop): 166.6 against 174.6 ms at 1M x 1M, alternated runs on one GPU.
+// The FP8 screen at K = 128 with 512-key buffers needs 165 VGPRs with the late index add
+// (append_quad) and so runs three per CU instead of two.
+template <class Op, int K, int CAP>
+constexpr bool screen_late() {
+  return (std::is_same<Op, OpBF16>::value && K <= 64 && CAP <= 256) ||
+         (std::is_same<Op, OpFP8>::value && K > 64 && K <= 128 && CAP == 512);
+}
 template <class Op, int K, int CAP>
 constexpr int screen_minb() {
   if constexpr (std::is_same<Op, OpBF16>::value && K <= 64 && CAP <= 256) return 4;
+  if constexpr (std::is_same<Op, OpFP8>::value && K > 64 && K <= 128 && CAP == 512) return 3;
   return (Op::kScaled && K >= 128) || CAP >= 1024 ? 2 : ScreenShape<Op, K>::MINB;
 }
 
@@ -329,7 +337,7 @@ __global__ void __launch_bounds__(256, (screen_minb<Op, K, CAP>())) k_screen(
     uint64_t *__restrict__ buf, uint64_t *__restrict__ out, int32_t *__restrict__ out_cnt) {
   typedef typename Op::Frag Frag;
   constexpr int QG = ScreenShape<Op, K>::QG;
-  constexpr bool kLate = screen_minb<Op, K, CAP>() == 4;  // see append_quad
+  constexpr bool kLate = screen_late<Op, K, CAP>();  // see append_quad
   constexpr int QPW = 32 * QG;                // queries per wave
   constexpr int QPB = ScreenShape<Op, K>::QPB;
   constexpr int HALFC = CAP / 2;              // keys per buffer half
