@@ -27,7 +27,7 @@ PEAK = {"bf16": 2.5e15, "fp32": 157.3e12, "fp8": 5.0e15, "fp64": 78.6e12}
 # HBM bytes per screen launch from FETCH_SIZE / WRITE_SIZE counter passes scaled by a 1 GiB
 # copy (profiles/screen_bf16_1m_traffic.md, profiles/screen_fp8_4m_traffic.md), keyed by
 # (workload, screen, pool)
-SCREEN_TRAFFIC = {("1m", "bf16", 64): 4.94e11, ("4m", "fp8", 128): 8.64e12}
+SCREEN_TRAFFIC = {("1m", "bf16", 64): 5.04e11, ("4m", "fp8", 128): 8.64e12}
 
 WORKLOADS = {
     # cells, genes, groups, density, chunk, dims, k, screen, pool, pca mode, recall queries
